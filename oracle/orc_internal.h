@@ -1,0 +1,44 @@
+/* orc_internal.h -- oracle workspace layout (TEST INFRASTRUCTURE ONLY).
+ * Holds what the reference keeps in the process globals of src/global_var.h. */
+#ifndef BQB_ORC_INTERNAL_H
+#define BQB_ORC_INTERNAL_H
+#include "oracle.h"
+
+#define ORC_BIG_NUMBER 1e+9
+
+struct orc_ws {
+    orc_params p;
+    int n0;                    /* vertices of the original problem (SP->n) */
+    int max_bundle, max_tri, max_pent, max_hept;
+    /* subproblem (PP) */
+    double *L;                 /* PP->L, n x n */
+    int n;
+    int NIneq, NPentIneq, NHeptaIneq, bundle;
+    /* primal */
+    double *X, *Z, *X_test, *X_bundle;
+    /* dual */
+    double *gamma, *dgamma, *gamma_test, *eta, *g, *lambda, *F;
+    double *G;                 /* m x bundle, ld = m (grown on demand) */
+    size_t G_cap;
+    double f;
+    /* cuts */
+    orc_tri *Cuts, *List;
+    orc_pent *Pent_Cuts, *Pent_List;
+    orc_hept *Hepta_Cuts, *Hepta_List;
+    /* node-local incumbent (Bab_LBGet / updateSolution) */
+    const double *SP_L;
+    double LB;
+    int *xbest;
+    int improved;
+    /* stats */
+    int ipm_calls, ipm_iters, status, max_cuts, max_bundle_seen;
+};
+
+int orc_ws_ensure_G(orc_ws *w, size_t need);
+/* bundle.c:9-239 on the workspace */
+int orc_bundle_method(orc_ws *w, double *t, int bdl_iter);
+/* bundle.c:242-275 on the workspace cut lists */
+double orc_ws_fct_eval(orc_ws *w, const double *gamma, double *X, double *g);
+/* heuristic.c:19-115; returns the value of the last GW round, x[n0-1] out */
+double orc_run_heuristic(orc_ws *w, const int *xfixed, const int *solX, int *x, uint64_t key);
+#endif

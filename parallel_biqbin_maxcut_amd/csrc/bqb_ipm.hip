@@ -1,0 +1,385 @@
+// bqb_ipm.hip -- batched FP64 primal-dual interior-point solve of the basic
+// Max-Cut SDP  max <L,X> s.t. diag(X) = e, X psd  for many independent
+// subproblems of different size (reference src/ipm_mc_pk.c:28-359).
+//
+// One workgroup bounds one subproblem and runs the whole predictor-corrector
+// loop without leaving the kernel.  Per iteration:
+//   Zi = inv(Diag(zd) - L)          symmetric sweep, registers
+//   Mi = inv(Zi o X)                symmetric sweep, registers
+//   dy1 = Mi (-e)
+//   dX1 = sym(Zi (-Diag(dy1) X) - X)          register-tile GEMM, LDS panels
+//   dy2 = Mi (mu diag(Zi) - (Zi o dX1) dy1)
+//   dX  = sym(Zi (-Diag(dy2) X - Diag(dy1) dX1) + mu Zi + dX1)
+//   alpha_p / alpha_d by Cholesky tests of X + a dX and Z + a Diag(dy),
+//   a <- 0.8 a on failure, then 0.95 a if a < 1
+//   X += alpha_p dX, y += alpha_d dy, Z += alpha_d Diag(dy),
+//   mu = <Z,X>/2n (x0.5 if alpha_p+alpha_d > 1.6, x0.2 if > 1.9)
+// until |e'y - <L,X>| <= 1e-2.  Z is kept as its diagonal zd (its off-diagonal
+// is -L and never changes, as in the reference where dZ is diagonal).
+//
+// Only one n x n register tile is live at a time; Zi, Mi and dX1/dX are kept
+// in the node's HBM slab between phases and streamed back (column-contiguous
+// panels for the GEMMs).
+//
+// The reference factors Zi o X twice with dposv; here it is inverted once and
+// both right-hand sides are matrix-vector products (same solution in exact
+// arithmetic).  Zi = inv(Z) uses the sweep instead of dpotrf+dpotri.
+#include <hip/hip_runtime.h>
+
+#include "bqb_dist.h"
+#include "bqb_kernels.h"
+
+namespace bqb {
+
+constexpr int IPM_MAX_ITERS = 500;
+constexpr int LS_MAX_TRIALS = 500;
+constexpr int KB = 8;   // GEMM panel depth
+
+// C += A * T for a symmetric A held in HBM (column-major slab, ld) and a T
+// whose row k is formed by the owners of column k from element (J, k) of
+// symmetric operands: T(k, J) = tfun(k, J + k*ld) (every operand is
+// symmetric, so row k equals column k).  Panels of KB columns of A and rows
+// of T are staged in LDS, double buffered: one barrier per panel.
+template <int P, int R, class TF>
+__device__ __forceinline__ void gemm_sym(double (&C)[R][R], const double *__restrict__ Ag, int ld,
+                                         int n, double *pan, const Thr &t, TF tfun)
+{
+    constexpr int NMAX = P * R;
+    int buf = 0;
+    for (int k0 = 0; k0 < n; k0 += KB) {
+        double *Ap = pan + buf * (2 * KB * NMAX);
+        double *Tp = Ap + KB * NMAX;
+        // KB*NMAX elements per panel, spread over all threads; column k of
+        // the slab is contiguous, so consecutive threads read consecutive rows.
+        for (int e = t.tid; e < KB * NMAX; e += P * P) {
+            const int kk = e / NMAX, J = e - kk * NMAX;
+            const int k = k0 + kk;
+            const size_t idx = (size_t)J + (size_t)k * ld;
+            const bool ok = k < n;
+            Ap[e] = ok ? Ag[idx] : 0.0;
+            Tp[e] = ok ? tfun(k, idx) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            double aI[R], tJ[R];
+#pragma unroll
+            for (int a = 0; a < R; ++a) aI[a] = Ap[q * NMAX + t.tr + P * a];
+#pragma unroll
+            for (int b = 0; b < R; ++b) tJ[b] = Tp[q * NMAX + t.tc + P * b];
+#pragma unroll
+            for (int a = 0; a < R; ++a)
+#pragma unroll
+                for (int b = 0; b < R; ++b) C[a][b] = fma(aI[a], tJ[b], C[a][b]);
+        }
+        buf ^= 1;
+    }
+    __syncthreads();
+}
+
+template <int P, int R>
+__global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
+{
+    constexpr int NMAX = P * R;
+    constexpr int NW = P * P / 64;
+    using L_ = Lds<P, R, KB>;
+    __shared__ double s_col[L_::COL];
+    __shared__ double s_scr[L_::UNION];
+    __shared__ double s_y[NMAX], s_zd[NMAX], s_dy1[NMAX], s_dy2[NMAX], s_v[NMAX], s_dg[NMAX];
+    __shared__ double s_red[NW];
+
+    const int node = blockIdx.x;
+    if (args.active && !args.active[node]) return;
+    const Thr t = thr_id<P>();
+    const int n = args.n[node];
+    const int ld = args.ld;
+    const size_t nn = (size_t)ld * ld;
+    const double *__restrict__ L = args.L + (size_t)node * nn;
+    double *__restrict__ X = args.X + (size_t)node * nn;
+    double *__restrict__ Wzi = args.work + (size_t)node * 3 * nn;     // inv(Z)
+    double *__restrict__ Wmi = Wzi + nn;                               // inv(Zi o X)
+    double *__restrict__ Wdx = Wmi + nn;                               // dX1, then dX
+    const Slab sL = make_slab<P>(L, ld, t), sX = make_slab<P>(X, ld, t);
+    const Slab sZi = make_slab<P>(Wzi, ld, t), sMi = make_slab<P>(Wmi, ld, t);
+    const Slab sDx = make_slab<P>(Wdx, ld, t);
+
+    if (n < 1 || n > NMAX || ld < NMAX) {
+        if (t.tid == 0) { args.status[node] = ERR_SIZE; args.iters[node] = 0; }
+        return;
+    }
+
+    double A[R][R];
+
+    // ---- initial point (ipm_mc_pk.c:65-99) -------------------------------
+    // y_j = 1.1 + sum_i |L_ij| (column sums: reduce over tr inside the wave)
+    dload<P, R>(A, sL);
+    {
+        double cs[R];
+#pragma unroll
+        for (int b = 0; b < R; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < R; ++a) s += fabs(A[a][b]);
+#pragma unroll
+            for (int o = 1; o < P; o <<= 1) s += __shfl_xor(s, o, 64);
+            cs[b] = s;
+        }
+        if (t.tr == 0) {
+#pragma unroll
+            for (int b = 0; b < R; ++b) {
+                const int J = t.tc + P * b;
+                s_y[J] = (J < n) ? cs[b] + 1.1 : 0.0;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+            if (t.tr == t.tc) s_dg[t.tr + P * a] = A[a][a];
+    }
+    __syncthreads();
+    for (int I = t.tid; I < NMAX; I += P * P) s_zd[I] = (I < n) ? s_y[I] - s_dg[I] : 0.0;
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+#pragma unroll
+        for (int b = 0; b < R; ++b) A[a][b] = (t.tr + P * a == t.tc + P * b && t.tr + P * a < n) ? 1.0 : 0.0;
+    dstore<P, R>(A, sX);                                // X = I
+    __syncthreads();
+
+    double phi, psi, mu;
+    {
+        double py = 0.0, pl = 0.0, pz = 0.0;
+        for (int I = t.tid; I < n; I += P * P) { py += s_y[I]; pl += s_dg[I]; pz += s_zd[I]; }
+        phi = wg_sum<P>(py, s_red, t);
+        psi = wg_sum<P>(pl, s_red, t);
+        mu = wg_sum<P>(pz, s_red, t) / (2.0 * n);
+    }
+    double gap = fabs(phi - psi);
+    int iter = 0, trials = 0, status = 0;
+
+    while (gap > 1e-2) {
+        if (iter >= IPM_MAX_ITERS) { status = ERR_IPM_ITERS; break; }
+        ++iter;
+
+        // ---- Zi = inv(Z) --------------------------------------------------
+        dload<P, R>(A, sL);
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+            for (int b = 0; b < R; ++b) {
+                const int I = t.tr + P * a, J = t.tc + P * b;
+                A[a][b] = (I == J) ? s_zd[I] + (-A[a][b]) : -A[a][b];
+            }
+        if (!sweep_inverse<P, R>(A, n, s_col, t)) { status = ERR_CHOL_Z; break; }
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+            if (t.tr == t.tc) s_dg[t.tr + P * a] = A[a][a];
+        dstore<P, R>(A, sZi);
+
+        // ---- Mi = inv(Zi o X), dy1 = Mi (-e) ------------------------------
+#pragma unroll
+        for (int b = 0; b < R; ++b)
+#pragma unroll
+            for (int a = 0; a < R; ++a) A[a][b] = A[a][b] * sld<P>(sX, a, b);
+        if (!sweep_inverse<P, R>(A, n, s_col, t)) { status = ERR_SCHUR; break; }
+        {
+            double part[R];
+#pragma unroll
+            for (int a = 0; a < R; ++a) {
+                double s = 0.0;
+#pragma unroll
+                for (int b = 0; b < R; ++b) s += A[a][b];
+                part[a] = s;
+            }
+            row_sums<P, R>(part, s_scr, s_v, t);
+            for (int I = t.tid; I < NMAX; I += P * P) s_dy1[I] = -s_v[I];
+        }
+        dstore<P, R>(A, sMi);
+        __syncthreads();
+
+        // ---- dX1 = sym(Zi * (-Diag(dy1) X) - X) ---------------------------
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+            for (int b = 0; b < R; ++b) A[a][b] = 0.0;
+        gemm_sym<P, R>(A, Wzi, ld, n, s_scr, t, [&](int k, size_t idx) {
+            return -s_dy1[k] * X[idx];
+        });
+#pragma unroll
+        for (int b = 0; b < R; ++b)
+#pragma unroll
+            for (int a = 0; a < R; ++a) A[a][b] = A[a][b] - sld<P>(sX, a, b);
+        symmetrize<P, R>(A, s_scr, t);
+        dstore<P, R>(A, sDx);
+
+        // ---- corrector rhs: mu diag(Zi) - (Zi o dX1) dy1 ------------------
+        {
+            double part[R];
+#pragma unroll
+            for (int a = 0; a < R; ++a) part[a] = 0.0;
+#pragma unroll
+            for (int b = 0; b < R; ++b) {
+                const double vj = s_dy1[t.tc + P * b];
+#pragma unroll
+                for (int a = 0; a < R; ++a) part[a] += sld<P>(sZi, a, b) * A[a][b] * vj;
+            }
+            row_sums<P, R>(part, s_scr, s_v, t);
+            for (int I = t.tid; I < NMAX; I += P * P) s_v[I] = (I < n) ? mu * s_dg[I] - s_v[I] : 0.0;
+            __syncthreads();
+        }
+        // dy2 = Mi * rhs (Mi streamed back)
+        {
+            double part[R];
+#pragma unroll
+            for (int a = 0; a < R; ++a) part[a] = 0.0;
+#pragma unroll
+            for (int b = 0; b < R; ++b) {
+                const double vj = s_v[t.tc + P * b];
+#pragma unroll
+                for (int a = 0; a < R; ++a) part[a] += sld<P>(sMi, a, b) * vj;
+            }
+            row_sums<P, R>(part, s_scr, s_dy2, t);
+        }
+
+        // ---- dX = sym(Zi * (-Diag(dy2) X - Diag(dy1) dX1) + mu Zi + dX1) --
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+            for (int b = 0; b < R; ++b) A[a][b] = 0.0;
+        gemm_sym<P, R>(A, Wzi, ld, n, s_scr, t, [&](int k, size_t idx) {
+            return (-s_dy2[k] * X[idx]) + (-s_dy1[k] * Wdx[idx]);
+        });
+#pragma unroll
+        for (int b = 0; b < R; ++b)
+#pragma unroll
+            for (int a = 0; a < R; ++a)
+                A[a][b] = (A[a][b] + mu * sld<P>(sZi, a, b)) + sld<P>(sDx, a, b);
+        symmetrize<P, R>(A, s_scr, t);
+        dstore<P, R>(A, sDx);                         // dX
+        for (int I = t.tid; I < NMAX; I += P * P) s_v[I] = s_dy2[I] + s_dy1[I];   // dy
+        __syncthreads();
+
+        // ---- primal line search: X + a dX psd -----------------------------
+        double alpha_p = 1.0;
+        {
+            int tr_ = 0;
+            bool ok = false;
+            while (tr_ < LS_MAX_TRIALS) {
+#pragma unroll
+                for (int b = 0; b < R; ++b)
+#pragma unroll
+                    for (int a = 0; a < R; ++a)
+                        A[a][b] = sld<P>(sX, a, b) + alpha_p * sld<P>(sDx, a, b);
+                ++tr_;
+                if (chol_test<P, R>(A, n, s_col, t)) { ok = true; break; }
+                alpha_p *= 0.8;
+            }
+            trials += tr_;
+            if (!ok) { status = ERR_LINESEARCH; break; }
+            if (alpha_p < 1.0) alpha_p *= 0.95;
+        }
+        // ---- dual line search: Z + a Diag(dy) psd --------------------------
+        double alpha_d = 1.0;
+        {
+            int tr_ = 0;
+            bool ok = false;
+            while (tr_ < LS_MAX_TRIALS) {
+                dload<P, R>(A, sL);
+#pragma unroll
+                for (int a = 0; a < R; ++a)
+#pragma unroll
+                    for (int b = 0; b < R; ++b) {
+                        const int I = t.tr + P * a, J = t.tc + P * b;
+                        A[a][b] = (I == J) ? s_zd[I] + alpha_d * s_v[I] : -A[a][b];
+                    }
+                ++tr_;
+                if (chol_test<P, R>(A, n, s_col, t)) { ok = true; break; }
+                alpha_d *= 0.8;
+            }
+            trials += tr_;
+            if (!ok) { status = ERR_LINESEARCH; break; }
+            if (alpha_d < 1.0) alpha_d *= 0.95;
+        }
+        if (args.alpha_trace && t.tid == 0 && iter <= args.trace_cap) {
+            args.alpha_trace[(size_t)node * 2 * args.trace_cap + 2 * (iter - 1)] = alpha_p;
+            args.alpha_trace[(size_t)node * 2 * args.trace_cap + 2 * (iter - 1) + 1] = alpha_d;
+        }
+
+        // ---- update -------------------------------------------------------
+#pragma unroll
+        for (int b = 0; b < R; ++b)
+#pragma unroll
+            for (int a = 0; a < R; ++a)
+                A[a][b] = sld<P>(sX, a, b) + alpha_p * sld<P>(sDx, a, b);
+        dstore<P, R>(A, sX);
+        for (int I = t.tid; I < n; I += P * P) {
+            s_y[I] = s_y[I] + alpha_d * s_v[I];
+            s_zd[I] = s_zd[I] + alpha_d * s_v[I];
+        }
+        __syncthreads();
+        {
+            double pz = 0.0, pl = 0.0;
+#pragma unroll
+            for (int b = 0; b < R; ++b)
+#pragma unroll
+                for (int a = 0; a < R; ++a) {
+                    const int I = t.tr + P * a, J = t.tc + P * b;
+                    const double l = sld<P>(sL, a, b);
+                    const double z = (I == J) ? s_zd[I] : -l;
+                    pz += z * A[a][b];
+                    pl += l * A[a][b];
+                }
+            double py = 0.0;
+            for (int I = t.tid; I < n; I += P * P) py += s_y[I];
+            mu = wg_sum<P>(pz, s_red, t) / (2.0 * n);
+            psi = wg_sum<P>(pl, s_red, t);
+            phi = wg_sum<P>(py, s_red, t);
+        }
+        if (alpha_p + alpha_d > 1.6) mu *= 0.5;
+        if (alpha_p + alpha_d > 1.9) mu *= 0.2;
+        gap = fabs(phi - psi);
+    }
+
+    if (t.tid == 0) {
+        args.phi[node] = phi;
+        args.iters[node] = iter;
+        args.trials[node] = trials;
+        args.status[node] = status;
+    }
+    if (args.y_out)
+        for (int I = t.tid; I < n; I += P * P) args.y_out[(size_t)node * args.ystride + I] = s_y[I];
+}
+
+template <int P, int R>
+static hipError_t launch_ipm(const IpmBatchArgs &a, int nodes, hipStream_t s)
+{
+    hipLaunchKernelGGL((ipm_batch_kernel<P, R>), dim3(nodes), dim3(P * P), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace bqb
+
+using namespace bqb;
+
+// Picks the register-tile shape for the largest n in the batch.
+extern "C" int bqb_ipm_tile_for(int nmax)
+{
+    if (nmax <= 64) return 1604;
+    if (nmax <= 80) return 1605;
+    if (nmax <= 96) return 1606;
+    if (nmax <= 112) return 1607;
+    if (nmax <= 128) return 3204;
+    if (nmax <= 160) return 3205;
+    return 0;
+}
+
+extern "C" hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int tile, hipStream_t s)
+{
+    switch (tile) {
+    case 1604: return launch_ipm<16, 4>(*a, nodes, s);
+    case 1605: return launch_ipm<16, 5>(*a, nodes, s);
+    case 1606: return launch_ipm<16, 6>(*a, nodes, s);
+    case 1607: return launch_ipm<16, 7>(*a, nodes, s);
+    case 3204: return launch_ipm<32, 4>(*a, nodes, s);
+    case 3205: return launch_ipm<32, 5>(*a, nodes, s);
+    default: return hipErrorInvalidValue;
+    }
+}

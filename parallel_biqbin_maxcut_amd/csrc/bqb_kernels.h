@@ -1,0 +1,56 @@
+// bqb_kernels.h -- argument blocks and launchers shared between the HIP
+// translation units and the C host code.  Plain C layout, no C++ types.
+#ifndef BQB_KERNELS_H
+#define BQB_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (same numbering as the CPU oracle).  The reference calls
+ * MPI_Abort(MPI_COMM_WORLD, 10) at the matching places. */
+enum {
+    ERR_OK = 0,
+    ERR_CHOL_Z = 1,        /* ipm_mc_pk.c:132-136 */
+    ERR_INV_Z = 2,
+    ERR_SCHUR = 3,         /* ipm_mc_pk.c:169-173, 226-230 */
+    ERR_IPM_ITERS = 4,     /* iteration cap of the otherwise unbounded IPM loop */
+    ERR_LINESEARCH = 5,    /* cap of the otherwise unbounded backtracking */
+    ERR_BUNDLE_FULL = 6,   /* bounding.c:312-315, bundle.c:140-143 */
+    ERR_QP = 7,            /* bundle.c:552-556 */
+    ERR_HEUR_CHOL = 8,     /* heuristic.c:70-74 */
+    ERR_SA_STAGES = 9,
+    ERR_ALLOC = 10,
+    ERR_SIZE = 11,         /* subproblem larger than the compiled tile */
+    ERR_CUTS_FULL = 12     /* device cut arena full */
+};
+
+/* Batched IPM: node b has size n[b] <= ld; its L and X are column-major
+ * ld x ld slabs at offset b*ld*ld (doubles), zero outside the leading n x n
+ * block (the kernel keeps X zero there).  work holds 3*ld*ld doubles per node. */
+typedef struct IpmBatchArgs {
+    const int *n;
+    const double *L;
+    double *X;
+    double *work;
+    int ld;
+    double *phi;
+    int *iters;
+    int *trials;
+    int *status;
+    const int *active;     /* optional mask (NULL = all) */
+    double *y_out;         /* optional [nodes*ystride] */
+    long ystride;
+    double *alpha_trace;   /* optional [nodes * 2 * trace_cap] */
+    int trace_cap;
+} IpmBatchArgs;
+
+int bqb_ipm_tile_for(int nmax);
+hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int tile, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

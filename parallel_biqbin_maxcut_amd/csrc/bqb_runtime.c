@@ -1,0 +1,207 @@
+/*
+ * bqb_runtime.c -- host side of the HIP bounding library, written in C.
+ *
+ * Device selection, the per-thread error string, and the host-pointer entry
+ * points for the batched IPM (used by the drop-in ipm_mc_pk and by tests).
+ * Device memory for a call is taken from a grow-only arena owned by the
+ * library (the reference's allocMemory/freeMemory, src/allocate_free.c:8-76,
+ * likewise sizes its scratch once per process).
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "bqb_kernels.h"
+#include "bqb_runtime.h"
+
+static char g_err[512];
+static int g_device = -1;
+
+void bqb_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+const char *bqb_last_error(void) { return g_err; }
+
+int bqb_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    bqb_set_error("%s: %s", what, hipGetErrorString(e));
+    return -1;
+}
+
+int bqb_init(int device)
+{
+    int count = 0;
+    if (bqb_check(hipGetDeviceCount(&count), "hipGetDeviceCount")) return -1;
+    if (count < 1) { bqb_set_error("no HIP device"); return -1; }
+    if (device < 0) device = 0;
+    if (device >= count) { bqb_set_error("device %d out of range (%d devices)", device, count); return -1; }
+    if (bqb_check(hipSetDevice(device), "hipSetDevice")) return -1;
+    g_device = device;
+    return 0;
+}
+
+int bqb_ensure_init(void)
+{
+    if (g_device >= 0) return hipSetDevice(g_device) == hipSuccess ? 0 : -1;
+    return bqb_init(0);
+}
+
+/* ---------------------------------------------------------------- arenas */
+int bqb_buf_reserve(bqb_buf *b, size_t bytes)
+{
+    if (b->cap >= bytes && b->ptr) return 0;
+    if (b->ptr) { hipFree(b->ptr); b->ptr = NULL; b->cap = 0; }
+    if (bytes == 0) return 0;
+    if (bqb_check(hipMalloc(&b->ptr, bytes), "hipMalloc")) return -1;
+    b->cap = bytes;
+    return 0;
+}
+
+void bqb_buf_free(bqb_buf *b)
+{
+    if (b->ptr) hipFree(b->ptr);
+    b->ptr = NULL;
+    b->cap = 0;
+}
+
+int bqb_hbuf_reserve(bqb_buf *b, size_t bytes)
+{
+    if (b->cap >= bytes && b->ptr) return 0;
+    if (b->ptr) { hipHostFree(b->ptr); b->ptr = NULL; b->cap = 0; }
+    if (bytes == 0) return 0;
+    if (bqb_check(hipHostMalloc(&b->ptr, bytes, hipHostMallocDefault), "hipHostMalloc")) return -1;
+    b->cap = bytes;
+    return 0;
+}
+
+void bqb_hbuf_free(bqb_buf *b)
+{
+    if (b->ptr) hipHostFree(b->ptr);
+    b->ptr = NULL;
+    b->cap = 0;
+}
+
+/* ---------------------------------------------------------------- IPM */
+static bqb_buf g_ipm_dev, g_ipm_host;
+
+int bqb_ipm_ld_for(int nmax)
+{
+    int tile = bqb_ipm_tile_for(nmax);
+    if (!tile) return 0;
+    int P = tile / 100, R = tile % 100;
+    return P * R;
+}
+
+int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
+                  int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap)
+{
+    if (nodes <= 0) return 0;
+    if (bqb_ensure_init()) return -1;
+    int nmax = 0;
+    for (int b = 0; b < nodes; ++b) {
+        if (n[b] < 1) { bqb_set_error("node %d: n = %d", b, n[b]); return -1; }
+        if (n[b] > nmax) nmax = n[b];
+    }
+    int tile = bqb_ipm_tile_for(nmax);
+    if (!tile) { bqb_set_error("n = %d exceeds the largest compiled IPM tile", nmax); return -1; }
+    const int ld = bqb_ipm_ld_for(nmax);
+    const size_t nn = (size_t)ld * ld;
+    if (trace_cap < 0) trace_cap = 0;
+
+    /* device layout: n | L slabs | X slabs | work (3 slabs/node) | phi | iters | trials | status | y | trace */
+    size_t off_n = 0;
+    size_t off_L = (off_n + sizeof(int) * nodes + 255) & ~(size_t)255;
+    size_t off_X = off_L + sizeof(double) * nn * nodes;
+    size_t off_W = off_X + sizeof(double) * nn * nodes;
+    size_t off_phi = off_W + sizeof(double) * 3 * nn * nodes;
+    size_t off_it = off_phi + sizeof(double) * nodes;
+    size_t off_tr = off_it + sizeof(int) * nodes;
+    size_t off_st = off_tr + sizeof(int) * nodes;
+    size_t off_y = (off_st + sizeof(int) * nodes + 255) & ~(size_t)255;
+    size_t off_a = off_y + sizeof(double) * ld * nodes;
+    size_t total = off_a + sizeof(double) * 2 * (size_t)trace_cap * nodes;
+    if (bqb_buf_reserve(&g_ipm_dev, total)) return -1;
+    size_t hbytes = off_W;                      /* n + L slabs + X slabs */
+    if (bqb_hbuf_reserve(&g_ipm_host, hbytes)) return -1;
+
+    char *h = (char *)g_ipm_host.ptr;
+    char *d = (char *)g_ipm_dev.ptr;
+    memcpy(h + off_n, n, sizeof(int) * nodes);
+    double *hL = (double *)(h + off_L);
+    memset(hL, 0, sizeof(double) * nn * nodes);
+    size_t src = 0;
+    for (int b = 0; b < nodes; ++b) {
+        for (int j = 0; j < n[b]; ++j)
+            memcpy(hL + b * nn + (size_t)j * ld, L + src + (size_t)j * n[b], sizeof(double) * n[b]);
+        src += (size_t)n[b] * n[b];
+    }
+    if (bqb_check(hipMemcpy(d, h, off_X, hipMemcpyHostToDevice), "hipMemcpy H2D")) return -1;
+    if (bqb_check(hipMemset(d + off_X, 0, off_phi - off_X), "hipMemset")) return -1;
+
+    IpmBatchArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = (const int *)(d + off_n);
+    a.L = (const double *)(d + off_L);
+    a.X = (double *)(d + off_X);
+    a.work = (double *)(d + off_W);
+    a.ld = ld;
+    a.phi = (double *)(d + off_phi);
+    a.iters = (int *)(d + off_it);
+    a.trials = (int *)(d + off_tr);
+    a.status = (int *)(d + off_st);
+    a.active = NULL;
+    a.y_out = (double *)(d + off_y);
+    a.ystride = ld;
+    a.alpha_trace = trace_cap ? (double *)(d + off_a) : NULL;
+    a.trace_cap = trace_cap;
+    if (bqb_check(bqb_launch_ipm_batch(&a, nodes, tile, 0), "ipm_batch_kernel launch")) return -1;
+    if (bqb_check(hipDeviceSynchronize(), "ipm_batch_kernel")) return -1;
+
+    double *hX = (double *)(h + off_L);        /* reuse the L staging area */
+    if (bqb_check(hipMemcpy(hX, d + off_X, sizeof(double) * nn * nodes, hipMemcpyDeviceToHost), "hipMemcpy D2H")) return -1;
+    size_t dst = 0;
+    for (int b = 0; b < nodes; ++b) {
+        if (X)
+            for (int j = 0; j < n[b]; ++j)
+                memcpy(X + dst + (size_t)j * n[b], hX + b * nn + (size_t)j * ld, sizeof(double) * n[b]);
+        dst += (size_t)n[b] * n[b];
+    }
+    if (phi && bqb_check(hipMemcpy(phi, d + off_phi, sizeof(double) * nodes, hipMemcpyDeviceToHost), "D2H phi")) return -1;
+    if (iters && bqb_check(hipMemcpy(iters, d + off_it, sizeof(int) * nodes, hipMemcpyDeviceToHost), "D2H iters")) return -1;
+    if (trials && bqb_check(hipMemcpy(trials, d + off_tr, sizeof(int) * nodes, hipMemcpyDeviceToHost), "D2H trials")) return -1;
+    if (status && bqb_check(hipMemcpy(status, d + off_st, sizeof(int) * nodes, hipMemcpyDeviceToHost), "D2H status")) return -1;
+    if (y_out) {
+        double *hy = malloc(sizeof(double) * ld * nodes);
+        if (!hy) { bqb_set_error("malloc"); return -1; }
+        if (bqb_check(hipMemcpy(hy, d + off_y, sizeof(double) * ld * nodes, hipMemcpyDeviceToHost), "D2H y")) { free(hy); return -1; }
+        size_t o = 0;
+        for (int b = 0; b < nodes; ++b) { memcpy(y_out + o, hy + (size_t)b * ld, sizeof(double) * n[b]); o += n[b]; }
+        free(hy);
+    }
+    if (alpha_trace && trace_cap &&
+        bqb_check(hipMemcpy(alpha_trace, d + off_a, sizeof(double) * 2 * trace_cap * nodes, hipMemcpyDeviceToHost), "D2H trace"))
+        return -1;
+    return 0;
+}
+
+/* Drop-in for the reference's ipm_mc_pk (src/ipm_mc_pk.c:28, biqbin.h:176):
+ * same arguments and meaning; failures abort with exit code 10 as the
+ * reference's MPI_Abort(MPI_COMM_WORLD, 10) does. */
+void ipm_mc_pk(double *L, int n, double *X, double *phi, int print)
+{
+    int it = 0, st = 0;
+    (void)print;
+    if (bqb_ipm_batch(1, &n, L, X, phi, &it, NULL, &st, NULL, NULL, 0) || st) {
+        fprintf(stderr, "ipm_mc_pk: %s (status %d)\n", st ? "IPM failed" : bqb_last_error(), st);
+        exit(10);
+    }
+}

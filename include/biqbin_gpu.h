@@ -50,8 +50,28 @@ void ipm_mc_pk(double *L, int n, double *X, double *phi, int print);
  * phi[b] = e'y (the dual value, a valid upper bound).  iters/trials/status/
  * y_out (concatenated, length sum n[b]) / alpha_trace (nodes x 2*trace_cap)
  * may be NULL.  Follows src/ipm_mc_pk.c:28-359 per matrix. */
+/* Padded leading dimension of the register tile chosen for subproblems up to
+ * nmax vertices (0 if nmax is too large). */
+int bqb_ipm_ld_for(int nmax);
+
 int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
                   int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap);
+
+/* Diagnostic: one matrix, stop after max_iters iterations and return the
+ * kernel's internal inv(Z) (Zi), inv(Zi o X) (Mi) and search direction dX of
+ * the last iteration (each n x n column-major; any pointer may be NULL), and
+ * in dbg_out (4*ld doubles, ld = padded size) the initial y, diag(Z),
+ * diag(L) and (phi, psi, mu). */
+int bqb_ipm_debug(int n, const double *L, int max_iters, double *X, double *Zi, double *Mi,
+                  double *dX, double *y, double *phi, int *iters, double *alpha_trace,
+                  double *dbg_out);
+
+/* Diagnostic: apply one workgroup-distributed primitive to an n x n matrix
+ * (column-major) with register tile `tile` (1604, 1607 or 3204 = P*100 + R):
+ * op 0 inverse by symmetric sweep, 1 lower Cholesky factor, 2 Cholesky
+ * positive-definiteness test, 3 symmetrisation (A + A')/2.  *ok receives
+ * the primitive's success flag. */
+int bqb_dist_selftest(int tile, int op, int n, const double *in, double *out, int *ok);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,13 @@ from . import _native as N
 
 
 def declare(L):
-    pass
+    L.bqb_ipm_debug.argtypes = [C.c_int, N.dptr, C.c_int, N.dptr, N.dptr, N.dptr, N.dptr, N.dptr,
+                                N.dptr, N.iptr, N.dptr, N.dptr]
+    L.bqb_ipm_debug.restype = C.c_int
+    L.bqb_dist_selftest.argtypes = [C.c_int, C.c_int, C.c_int, N.dptr, N.dptr, N.iptr]
+    L.bqb_dist_selftest.restype = C.c_int
+    L.bqb_ipm_ld_for.argtypes = [C.c_int]
+    L.bqb_ipm_ld_for.restype = C.c_int
 
 
 def _d(a):
@@ -53,3 +59,30 @@ def ipm_batch(Ls, trace_cap=0):
         oy += n
     traces = [tr[b * 2 * trace_cap: b * 2 * trace_cap + 2 * iters[b]] for b in range(nodes)] if trace_cap else None
     return dict(X=Xs, phi=phi, iters=iters, trials=trials, status=status, y=ys, trace=traces)
+
+
+def ipm_debug(L, max_iters):
+    """Internal state after `max_iters` IPM iterations (diagnostic entry)."""
+    L = np.asfortranarray(L, dtype=np.float64)
+    n = L.shape[0]
+    X, Zi, Mi, dX = (np.zeros((n, n), order="F") for _ in range(4))
+    y = np.zeros(n)
+    phi = C.c_double()
+    it = C.c_int()
+    tr = np.zeros(2 * max_iters)
+    dbg = np.zeros(4 * 256)
+    N.check(N.lib().bqb_ipm_debug(n, _d(L), max_iters, _d(X), _d(Zi), _d(Mi), _d(dX), _d(y),
+                                  C.byref(phi), C.byref(it), _d(tr), _d(dbg)))
+    ld = N.lib().bqb_ipm_ld_for(n)
+    return dict(X=X, Zi=Zi, Mi=Mi, dX=dX, y=y, phi=phi.value, iters=it.value, trace=tr,
+                y0=dbg[:n], zd0=dbg[ld:ld + n], dgL=dbg[2 * ld:2 * ld + n], scal=dbg[3 * ld:3 * ld + 3])
+
+
+def dist_selftest(tile, op, A):
+    """Apply one distributed primitive (diagnostic entry); returns (out, ok)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    n = A.shape[0]
+    out = np.zeros((n, n), order="F")
+    ok = C.c_int()
+    N.check(N.lib().bqb_dist_selftest(tile, op, n, _d(A), _d(out), C.byref(ok)))
+    return out, ok.value

@@ -182,6 +182,7 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
 {
     constexpr int NMAX = P * R;
     int buf = 0;
+    __syncthreads();   // colbuf may still be read by the previous pivot loop
     bool ok = true;
     static_for<0, R>([&](auto kbc) {
         constexpr int kb = decltype(kbc)::value;
@@ -236,6 +237,7 @@ __device__ __forceinline__ bool chol_test(double (&A)[R][R], int n, double *colb
 {
     constexpr int NMAX = P * R;
     int buf = 0;
+    __syncthreads();   // colbuf may still be read by the previous pivot loop
     bool ok = true;
     static_for<0, R>([&](auto kbc) {
         constexpr int kb = decltype(kbc)::value;
@@ -273,6 +275,7 @@ __device__ __forceinline__ bool chol_lower(double (&A)[R][R], int n, double *col
 {
     constexpr int NMAX = P * R;
     int buf = 0;
+    __syncthreads();   // colbuf may still be read by the previous pivot loop
     bool ok = true;
     static_for<0, R>([&](auto kbc) {
         constexpr int kb = decltype(kbc)::value;

@@ -41,7 +41,7 @@ constexpr int KB = 8;   // GEMM panel depth
 // symmetric, so row k equals column k).  Panels of KB columns of A and rows
 // of T are staged in LDS, double buffered: one barrier per panel.
 template <int P, int R, class TF>
-__device__ __forceinline__ void gemm_sym(double (&C)[R][R], const double *__restrict__ Ag, int ld,
+__device__ __forceinline__ void gemm_sym(double (&C)[R][R], const double *Ag, int ld,
                                          int n, double *pan, const Thr &t, TF tfun)
 {
     constexpr int NMAX = P * R;
@@ -94,11 +94,11 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
     const int n = args.n[node];
     const int ld = args.ld;
     const size_t nn = (size_t)ld * ld;
-    const double *__restrict__ L = args.L + (size_t)node * nn;
-    double *__restrict__ X = args.X + (size_t)node * nn;
-    double *__restrict__ Wzi = args.work + (size_t)node * 3 * nn;     // inv(Z)
-    double *__restrict__ Wmi = Wzi + nn;                               // inv(Zi o X)
-    double *__restrict__ Wdx = Wmi + nn;                               // dX1, then dX
+    const double *L = args.L + (size_t)node * nn;
+    double *X = args.X + (size_t)node * nn;
+    double *Wzi = args.work + (size_t)node * 3 * nn;     // inv(Z)
+    double *Wmi = Wzi + nn;                              // inv(Zi o X)
+    double *Wdx = Wmi + nn;                              // dX1, then dX
     const Slab sL = make_slab<P>(L, ld, t), sX = make_slab<P>(X, ld, t);
     const Slab sZi = make_slab<P>(Wzi, ld, t), sMi = make_slab<P>(Wmi, ld, t);
     const Slab sDx = make_slab<P>(Wdx, ld, t);
@@ -154,9 +154,22 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
     }
     double gap = fabs(phi - psi);
     int iter = 0, trials = 0, status = 0;
+    if (args.dbg) {
+        double *dbg = args.dbg + (size_t)node * 4 * ld;
+        for (int I = t.tid; I < NMAX; I += P * P) {
+            dbg[I] = s_y[I];
+            dbg[ld + I] = s_zd[I];
+            dbg[2 * ld + I] = s_dg[I];
+        }
+        if (t.tid == 0) { dbg[3 * ld] = phi; dbg[3 * ld + 1] = psi; dbg[3 * ld + 2] = mu; }
+    }
 
+    const int max_iters = args.max_iters > 0 ? args.max_iters : IPM_MAX_ITERS;
     while (gap > 1e-2) {
-        if (iter >= IPM_MAX_ITERS) { status = ERR_IPM_ITERS; break; }
+        if (iter >= max_iters) {
+            if (args.max_iters <= 0) status = ERR_IPM_ITERS;
+            break;
+        }
         ++iter;
 
         // ---- Zi = inv(Z) --------------------------------------------------
@@ -166,7 +179,7 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
 #pragma unroll
             for (int b = 0; b < R; ++b) {
                 const int I = t.tr + P * a, J = t.tc + P * b;
-                A[a][b] = (I == J) ? s_zd[I] + (-A[a][b]) : -A[a][b];
+                A[a][b] = (I == J) ? s_zd[I] : -A[a][b];   // Z = Diag(zd) - offdiag(L)
             }
         if (!sweep_inverse<P, R>(A, n, s_col, t)) { status = ERR_CHOL_Z; break; }
 #pragma unroll

@@ -45,6 +45,8 @@ typedef struct IpmBatchArgs {
     long ystride;
     double *alpha_trace;   /* optional [nodes * 2 * trace_cap] */
     int trace_cap;
+    int max_iters;         /* 0: default cap; k > 0: stop after k iterations */
+    double *dbg;           /* optional [nodes * 4 * ld]: initial y, zd, diag(L), and phi/psi/mu */
 } IpmBatchArgs;
 
 int bqb_ipm_tile_for(int nmax);

@@ -101,8 +101,40 @@ int bqb_ipm_ld_for(int nmax)
     return P * R;
 }
 
+static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, double *phi,
+                          int *iters, int *trials, int *status, double *y_out, double *alpha_trace,
+                          int trace_cap, int max_iters, double *work_out);
+
 int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
                   int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap)
+{
+    return ipm_batch_impl(nodes, n, L, X, phi, iters, trials, status, y_out, alpha_trace, trace_cap, 0, NULL);
+}
+
+int bqb_ipm_debug(int n, const double *L, int max_iters, double *X, double *Zi, double *Mi,
+                  double *dX, double *y, double *phi, int *iters, double *alpha_trace, double *dbg_out)
+{
+    int ld = bqb_ipm_ld_for(n);
+    if (!ld) { bqb_set_error("n too large"); return -1; }
+    double *w = malloc(sizeof(double) * (3 * (size_t)ld * ld + 4 * (size_t)ld));
+    if (!w) { bqb_set_error("malloc"); return -1; }
+    int st = 0;
+    int rc = ipm_batch_impl(1, &n, L, X, phi, iters, NULL, &st, y, alpha_trace, max_iters, max_iters, w);
+    if (rc == 0) {
+        double *outs[3] = {Zi, Mi, dX};
+        for (int m = 0; m < 3; ++m)
+            if (outs[m])
+                for (int j = 0; j < n; ++j)
+                    memcpy(outs[m] + (size_t)j * n, w + (size_t)m * ld * ld + (size_t)j * ld, sizeof(double) * n);
+        if (dbg_out) memcpy(dbg_out, w + 3 * (size_t)ld * ld, sizeof(double) * 4 * ld);
+    }
+    free(w);
+    return rc;
+}
+
+static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, double *phi,
+                          int *iters, int *trials, int *status, double *y_out, double *alpha_trace,
+                          int trace_cap, int max_iters, double *work_out)
 {
     if (nodes <= 0) return 0;
     if (bqb_ensure_init()) return -1;
@@ -128,7 +160,8 @@ int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *p
     size_t off_st = off_tr + sizeof(int) * nodes;
     size_t off_y = (off_st + sizeof(int) * nodes + 255) & ~(size_t)255;
     size_t off_a = off_y + sizeof(double) * ld * nodes;
-    size_t total = off_a + sizeof(double) * 2 * (size_t)trace_cap * nodes;
+    size_t off_dbg = off_a + sizeof(double) * 2 * (size_t)trace_cap * nodes;
+    size_t total = off_dbg + (work_out ? sizeof(double) * 4 * ld * nodes : 0);
     if (bqb_buf_reserve(&g_ipm_dev, total)) return -1;
     size_t hbytes = off_W;                      /* n + L slabs + X slabs */
     if (bqb_hbuf_reserve(&g_ipm_host, hbytes)) return -1;
@@ -163,6 +196,8 @@ int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *p
     a.ystride = ld;
     a.alpha_trace = trace_cap ? (double *)(d + off_a) : NULL;
     a.trace_cap = trace_cap;
+    a.max_iters = max_iters;
+    a.dbg = work_out ? (double *)(d + off_dbg) : NULL;
     if (bqb_check(bqb_launch_ipm_batch(&a, nodes, tile, 0), "ipm_batch_kernel launch")) return -1;
     if (bqb_check(hipDeviceSynchronize(), "ipm_batch_kernel")) return -1;
 
@@ -187,6 +222,12 @@ int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *p
         for (int b = 0; b < nodes; ++b) { memcpy(y_out + o, hy + (size_t)b * ld, sizeof(double) * n[b]); o += n[b]; }
         free(hy);
     }
+    if (work_out &&
+        bqb_check(hipMemcpy(work_out, d + off_W, sizeof(double) * 3 * nn * nodes, hipMemcpyDeviceToHost), "D2H work"))
+        return -1;
+    if (work_out &&
+        bqb_check(hipMemcpy(work_out + 3 * nn * nodes, d + off_dbg, sizeof(double) * 4 * ld * nodes, hipMemcpyDeviceToHost), "D2H dbg"))
+        return -1;
     if (alpha_trace && trace_cap &&
         bqb_check(hipMemcpy(alpha_trace, d + off_a, sizeof(double) * 2 * trace_cap * nodes, hipMemcpyDeviceToHost), "D2H trace"))
         return -1;
@@ -204,4 +245,36 @@ void ipm_mc_pk(double *L, int n, double *X, double *phi, int print)
         fprintf(stderr, "ipm_mc_pk: %s (status %d)\n", st ? "IPM failed" : bqb_last_error(), st);
         exit(10);
     }
+}
+
+/* ---------------------------------------------------------------- self-test */
+hipError_t bqb_launch_dist_selftest(int tile, int op, int n, int ld, const double *in, double *out,
+                                    int *flag);
+
+int bqb_dist_selftest(int tile, int op, int n, const double *in, double *out, int *ok)
+{
+    if (bqb_ensure_init()) return -1;
+    const int P = tile / 100, R = tile % 100;
+    const int ld = P * R;
+    if (n < 1 || n > ld) { bqb_set_error("n out of range for tile"); return -1; }
+    size_t bytes = sizeof(double) * (size_t)ld * ld;
+    double *h = calloc((size_t)ld * ld, sizeof(double));
+    void *d = NULL;
+    if (!h) { bqb_set_error("calloc"); return -1; }
+    for (int j = 0; j < n; ++j) memcpy(h + (size_t)j * ld, in + (size_t)j * n, sizeof(double) * n);
+    int rc = -1;
+    if (bqb_check(hipMalloc(&d, 2 * bytes + 256), "hipMalloc")) goto out;
+    if (bqb_check(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice), "H2D")) goto out;
+    if (bqb_check(hipMemset((char *)d + bytes, 0, bytes + 256), "memset")) goto out;
+    if (bqb_check(bqb_launch_dist_selftest(tile, op, n, ld, (const double *)d, (double *)((char *)d + bytes),
+                                           (int *)((char *)d + 2 * bytes)), "selftest launch")) goto out;
+    if (bqb_check(hipDeviceSynchronize(), "selftest")) goto out;
+    if (bqb_check(hipMemcpy(h, (char *)d + bytes, bytes, hipMemcpyDeviceToHost), "D2H")) goto out;
+    if (bqb_check(hipMemcpy(ok, (char *)d + 2 * bytes, sizeof(int), hipMemcpyDeviceToHost), "D2H")) goto out;
+    for (int j = 0; j < n; ++j) memcpy(out + (size_t)j * n, h + (size_t)j * ld, sizeof(double) * n);
+    rc = 0;
+out:
+    if (d) hipFree(d);
+    free(h);
+    return rc;
 }

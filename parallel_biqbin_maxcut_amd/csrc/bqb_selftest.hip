@@ -1,0 +1,45 @@
+// bqb_selftest.hip -- unit entry for the workgroup-distributed primitives of
+// bqb_dist.h: one workgroup loads an n x n matrix, applies one primitive and
+// stores the result.  Used by the tests to check each primitive against a
+// dense reference independently of the solvers built on top of them.
+#include <hip/hip_runtime.h>
+
+#include "bqb_dist.h"
+#include "bqb_kernels.h"
+
+namespace bqb {
+
+template <int P, int R>
+__global__ __launch_bounds__(P * P) void dist_selftest_kernel(int op, int n, int ld, const double *in,
+                                                              double *out, int *flag)
+{
+    constexpr int NMAX = P * R;
+    __shared__ double s_col[2 * NMAX];
+    __shared__ double s_T[NMAX * (P + 1)];
+    const Thr t = thr_id<P>();
+    const Slab si = make_slab<P>(in, ld, t), so = make_slab<P>(out, ld, t);
+    double A[R][R];
+    dload<P, R>(A, si);
+    bool ok = true;
+    if (op == 0) ok = sweep_inverse<P, R>(A, n, s_col, t);
+    else if (op == 1) ok = chol_lower<P, R>(A, n, s_col, t);
+    else if (op == 2) ok = chol_test<P, R>(A, n, s_col, t);
+    else if (op == 3) symmetrize<P, R>(A, s_T, t);
+    dstore<P, R>(A, so);
+    if (t.tid == 0) *flag = ok ? 1 : 0;
+}
+
+}  // namespace bqb
+
+extern "C" hipError_t bqb_launch_dist_selftest(int tile, int op, int n, int ld, const double *in,
+                                               double *out, int *flag)
+{
+    using namespace bqb;
+    switch (tile) {
+    case 1604: hipLaunchKernelGGL((dist_selftest_kernel<16, 4>), dim3(1), dim3(256), 0, 0, op, n, ld, in, out, flag); break;
+    case 1607: hipLaunchKernelGGL((dist_selftest_kernel<16, 7>), dim3(1), dim3(256), 0, 0, op, n, ld, in, out, flag); break;
+    case 3204: hipLaunchKernelGGL((dist_selftest_kernel<32, 4>), dim3(1), dim3(1024), 0, 0, op, n, ld, in, out, flag); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}

@@ -74,3 +74,40 @@ def test_ipm_batch_closed_form_complete_graph():
     Lap = np.diag(adj.sum(1)) - adj
     r = api.ipm_batch([Lap / 4])
     assert n * n / 4 - 1e-9 <= r["phi"][0] <= n * n / 4 + 1e-2
+
+
+def _numpy_first_iteration(L):
+    """One predictor-corrector step of src/ipm_mc_pk.c in numpy (dense solves)."""
+    n = L.shape[0]
+    y = np.abs(L).sum(0) + 1.1
+    X = np.eye(n)
+    Z = np.diag(y) - L
+    mu = np.sum(Z * X) / (2 * n)
+    Zi = np.linalg.inv(Z)
+    M = Zi * X
+    dy1 = np.linalg.solve(M, -np.ones(n))
+    dX1 = Zi @ (-np.diag(dy1) @ X) - X
+    dX1 = 0.5 * (dX1 + dX1.T)
+    rhs = mu * np.diag(Zi) - (Zi * dX1) @ dy1
+    dy2 = np.linalg.solve(M, rhs)
+    dX = Zi @ (-np.diag(dy2) @ X - np.diag(dy1) @ dX1) + mu * Zi + dX1
+    dX = 0.5 * (dX + dX.T)
+    return Zi, np.linalg.inv(M), dX, dy1 + dy2
+
+
+@pytest.mark.parametrize("n0", [20, 70, 100])
+def test_ipm_first_iteration_internals(n0):
+    L = sub_matrices(n0, 0.4, "pm1", 5, 1, 0)[0]
+    Zi, Mi, dX, dy = _numpy_first_iteration(L)
+    r = api.ipm_debug(L, 1)
+    assert r["iters"] == 1
+    y0 = np.abs(L).sum(0) + 1.1
+    assert np.allclose(r["dgL"], np.diag(L))
+    assert np.allclose(r["y0"], y0, rtol=1e-14), (r["y0"][:6], y0[:6])
+    assert np.allclose(r["zd0"], y0 - np.diag(L), rtol=1e-14)
+    assert np.allclose(r["scal"], [y0.sum(), np.trace(L), np.sum(y0 - np.diag(L)) / (2 * L.shape[0])])
+    assert np.max(np.abs(r["Zi"] - Zi)) < 1e-10 * np.max(np.abs(Zi))
+    assert np.max(np.abs(r["Mi"] - Mi)) < 1e-8 * np.max(np.abs(Mi))
+    assert np.max(np.abs(r["dX"] - dX)) < 1e-8 * np.max(np.abs(dX))
+    _, _, _, _, tr = O.ipm(L, trace_cap=400)
+    assert np.allclose(r["trace"][:2], tr[:2], atol=1e-12)

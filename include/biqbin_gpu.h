@@ -17,9 +17,38 @@
 #ifndef BIQBIN_GPU_H
 #define BIQBIN_GPU_H
 
+#include <stddef.h>
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* ---- types shared with the reference (binary compatible) --------------- */
+/* BiqBin parameters: same fields, order, types and meaning as the
+ * reference's PARAM_FIELDS X-macro (src/biqbin.h:49-78). */
+typedef struct BiqBinParameters {
+    int init_bundle_iter;
+    int max_bundle_iter;
+    int triag_iter;
+    int pent_iter;
+    int hept_iter;
+    int max_outer_iter;
+    int extra_iter;
+    double violated_TriIneq;
+    int TriIneq;
+    int adjust_TriIneq;
+    int PentIneq;
+    int HeptaIneq;
+    int Pent_Trials;
+    int Hepta_Trials;
+    int include_Pent;
+    int include_Hepta;
+    int root;
+    int use_diff;
+    int time_limit;
+    int branchingStrategy;
+} BiqBinParameters;
 
 /* ---- status codes per subproblem ------------------------------------- */
 #define BQB_OK 0
@@ -56,6 +85,53 @@ int bqb_ipm_ld_for(int nmax);
 
 int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
                   int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap);
+
+/* ---- batch: the full node bound (SDPbound) ----------------------------- */
+/* A context holds the device arenas of up to max_nodes B&B nodes of one
+ * instance: SP_L is the reference's SP->L (n0 x n0, column-major, built as
+ * in src/process_input.c:259-344), params the BiqBin parameters.  Capacities
+ * <= 0 take the defaults (bundle 64, triangle cuts 8192, pentagonal and
+ * heptagonal cuts 2048 each). */
+typedef struct bqb_ctx bqb_ctx;
+bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BiqBinParameters *params, int max_nodes,
+                        int max_bundle, int max_tri, int max_pent, int max_hept);
+void bqb_ctx_destroy(bqb_ctx *c);
+size_t bqb_ctx_bytes(const bqb_ctx *c);
+int bqb_ctx_ld(const bqb_ctx *c);
+double bqb_ctx_last_ms(const bqb_ctx *c);
+
+/* Bounds `nodes` B&B nodes with the algorithm of src/bounding.c:24-342
+ * (Evaluate = createSubproblem + SDPbound, src/evaluate.c:11-20).
+ * Per node b: xfixed/solX rows of length n0-1 (src/biqbin_cpp_api.h:54-63),
+ * LB[b] the incumbent value at batch start, diff[b] the root gap used by the
+ * give-up test, is_root[b] the reference's rank == 0 semantics, seeds[b] the
+ * node's random stream.  Outputs per node: bound (f + fixedvalue), fracsol
+ * (n0-1), xbest (n0-1, best heuristic cut; valid if improved), LB_out
+ * (LB raised by the node's own heuristic), diff_out, status, and 8 stats
+ * ints (prune, giveup, outer iterations, IPM calls, IPM iterations, max cuts,
+ * max bundle size, n).  Any output pointer may be NULL. */
+int bqb_bound_batch(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
+                    const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
+                    double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
+                    int *status, int *stats);
+
+/* Stepwise use of a context (tests and diagnostics): load a batch, run one
+ * phase (0-11 control phases; 100 basic IPM, 101 IPM on L0, 102 heuristic,
+ * 103 triangle update, 104/105 pentagonal/heptagonal update, 106 op_Bt
+ * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP,
+ * 109 bundle step), run the whole bound, read results, and copy one node's
+ * device buffer (ids documented in bqb_sdpbound.c) to or from the host. */
+int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
+                 const double *diff, const int *is_root, const uint64_t *seeds);
+int bqb_ctx_run(bqb_ctx *c, int phase, int arg);
+int bqb_ctx_bound(bqb_ctx *c);
+int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, double *LB_out,
+                    int *improved, double *diff_out, int *status, int *stats);
+int bqb_ctx_download(bqb_ctx *c, int id, int node, void *host, size_t bytes);
+int bqb_ctx_upload(bqb_ctx *c, int id, int node, const void *host, size_t bytes);
+int bqb_ctx_sync(bqb_ctx *c);
+void bqb_ctx_profile_ipm(bqb_ctx *c, int on);
+double bqb_ctx_ipm_ms(const bqb_ctx *c);
 
 /* Diagnostic: one matrix, stop after max_iters iterations and return the
  * kernel's internal inv(Z) (Zi), inv(Zi o X) (Mi) and search direction dX of

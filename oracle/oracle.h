@@ -212,6 +212,11 @@ void orc_bab_solve(const double *SP_L, int n0, const orc_params *p, uint64_t see
 
 void orc_params_default(orc_params *p);
 
+/* Test helper: the heuristic of one node on a given X (see orc_heur.c). */
+double orc_heuristic_on(const double *SP_L, int n0, const int *xfixed, const int *solX,
+                        const double *PP_L, const double *X, int n, uint64_t key, int *x_out,
+                        int *status);
+
 /* Bound many independent nodes with up to nthreads threads (cpu baseline). */
 void orc_sdpbound_many(const double *SP_L, int n0, const orc_params *p, int count,
                        const int *xfixed, const int *solX, const double *LB,

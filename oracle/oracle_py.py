@@ -114,6 +114,8 @@ def lib():
         L.orc_key.restype = C.c_uint64
         L.orc_draw.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_draw.restype = C.c_uint64
+        L.orc_heuristic_on.argtypes = [dptr, C.c_int, iptr, iptr, dptr, dptr, C.c_int, C.c_uint64, iptr, iptr]
+        L.orc_heuristic_on.restype = C.c_double
         L.orc_chol_upper.argtypes = [dptr, C.c_int]
         L.orc_chol_upper.restype = C.c_int
         _LIB = L
@@ -274,3 +276,40 @@ def bab(SP_L, params, seed=2020, batch=1):
     out = BabOut()
     lib().orc_bab_solve(_d(SP_L), n0, C.byref(params), seed, batch, _i(sol), C.byref(out))
     return out, sol
+
+
+def heuristic_on(SP_L, xfixed, solX, PL, X, key):
+    SP_L = f64(SP_L)
+    n0 = SP_L.shape[0]
+    PL = f64(PL)
+    X = f64(X)
+    xf = np.ascontiguousarray(xfixed, dtype=np.int32)
+    sx = np.ascontiguousarray(solX, dtype=np.int32)
+    x = np.zeros(n0 - 1, dtype=np.int32)
+    st = C.c_int()
+    v = lib().orc_heuristic_on(_d(SP_L), n0, _i(xf), _i(sx), _d(PL), _d(X), PL.shape[0], key, _i(x), C.byref(st))
+    return v, x, st.value
+
+
+def key(a, b):
+    return lib().orc_key(a, b)
+
+
+def pent_sep(X, trials, cap, k):
+    X = f64(X)
+    n = X.shape[0]
+    lst = (Pent * max(cap, 1))()
+    size = C.c_int()
+    st = C.c_int()
+    mn = lib().orc_get_violated_pent(_d(X), n, trials, cap, lst, C.byref(size), k, C.byref(st))
+    return mn, [(lst[c].type, [lst[c].perm[a] for a in range(5)], lst[c].value) for c in range(size.value)], st.value
+
+
+def hept_sep(X, trials, cap, k):
+    X = f64(X)
+    n = X.shape[0]
+    lst = (Hept * max(cap, 1))()
+    size = C.c_int()
+    st = C.c_int()
+    mn = lib().orc_get_violated_hept(_d(X), n, trials, cap, lst, C.byref(size), k, C.byref(st))
+    return mn, [(lst[c].type, [lst[c].perm[a] for a in range(7)], lst[c].value) for c in range(size.value)], st.value

@@ -170,19 +170,19 @@ int orc_lambda_eta(int m, int k, const double *zeta, const double *G, int ldG,
     double *tmp = calloc(m > 0 ? m : 1, sizeof(double));
     double *c = calloc(k, sizeof(double));
     if (!Q || !tmp || !c) { free(Q); free(tmp); free(c); return ORC_ERR_ALLOC; }
-    /* Q = t * G'G (dsyrk) */
+    /* Q = t * G'G (dsyrk; with m = 0 it yields Q = 0) */
     for (int j = 0; j < k; ++j)
         for (int i = j; i < k; ++i) {
             double s = 0.0;
             for (int l = 0; l < m; ++l) s += G[l + i * ldG] * G[l + j * ldG];
-            Q[i + j * k] = Q[j + i * k] = *t * s;
+            Q[i + j * k] = Q[j + i * k] = (m > 0) ? *t * s : 0.0;
         }
     while (!done) {
         ++cnt;
-        for (int j = 0; j < k; ++j) {          /* c = zeta - t G' eta */
+        for (int j = 0; j < k; ++j) {          /* c = zeta - t G' eta (dgemv returns early if m = 0) */
             double s = 0.0;
             for (int l = 0; l < m; ++l) s += G[l + j * ldG] * eta[l];
-            c[j] = zeta[j] + (-(*t)) * s;
+            c[j] = (m > 0) ? zeta[j] + (-(*t)) * s : zeta[j];
         }
         status = orc_solve_lambda(k, Q, c, lambda);
         if (status) break;
@@ -270,7 +270,8 @@ int orc_bundle_method(orc_ws *w, double *t, int bdl_iter)
         for (int j = 0; j < k; ++j) {           /* zeta = -F - G' gamma */
             double s = 0.0;
             for (int l = 0; l < m; ++l) s += w->G[l + (size_t)j * m] * w->gamma[l];
-            zeta[j] = -w->F[j] + (-1.0) * s;
+            /* with m = 0 dgemv returns before scaling, leaving zeta = F */
+            zeta[j] = (m > 0) ? -w->F[j] + (-1.0) * s : w->F[j];
         }
         int st = orc_lambda_eta(m, k, zeta, w->G, m, w->gamma, w->dgamma, w->lambda, w->eta, t);
         if (st) { w->status = st; break; }

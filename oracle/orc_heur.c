@@ -142,3 +142,25 @@ double orc_run_heuristic(orc_ws *w, const int *xfixed, const int *solX, int *x, 
     free(xh); free(temp_x);
     return heur_val;
 }
+
+/* Test helper: run the heuristic of one node on a given X (n x n) and
+ * subproblem matrix PP_L; x_out[n0-1] receives the last round's best cut and
+ * the return value is its value.  Stream key as the node bound uses it. */
+double orc_heuristic_on(const double *SP_L, int n0, const int *xfixed, const int *solX,
+                        const double *PP_L, const double *X, int n, uint64_t key, int *x_out,
+                        int *status)
+{
+    orc_params p;
+    orc_params_default(&p);
+    orc_ws *w = orc_ws_create(n0, 4, 4, 4, 4, &p);
+    w->SP_L = SP_L;
+    w->n = n;
+    memcpy(w->L, PP_L, sizeof(double) * n * n);
+    memcpy(w->X, X, sizeof(double) * n * n);
+    w->status = 0;
+    orc_run_heuristic(w, xfixed, solX, x_out, key);
+    double v = orc_evaluate_solution(SP_L, n0, x_out);
+    if (status) *status = w->status;
+    orc_ws_destroy(w);
+    return v;
+}

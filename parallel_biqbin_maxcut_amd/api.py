@@ -15,6 +15,7 @@ def declare(L):
     L.bqb_dist_selftest.restype = C.c_int
     L.bqb_ipm_ld_for.argtypes = [C.c_int]
     L.bqb_ipm_ld_for.restype = C.c_int
+    _declare_ctx(L)
 
 
 def _d(a):
@@ -86,3 +87,164 @@ def dist_selftest(tile, op, A):
     ok = C.c_int()
     N.check(N.lib().bqb_dist_selftest(tile, op, n, _d(A), _d(out), C.byref(ok)))
     return out, ok.value
+
+
+# ---------------------------------------------------------------- batch bound
+class BiqBinParameters(C.Structure):
+    """The reference's PARAM_FIELDS layout (src/biqbin.h:49-78)."""
+    _fields_ = [
+        ("init_bundle_iter", C.c_int), ("max_bundle_iter", C.c_int), ("triag_iter", C.c_int),
+        ("pent_iter", C.c_int), ("hept_iter", C.c_int), ("max_outer_iter", C.c_int),
+        ("extra_iter", C.c_int), ("violated_TriIneq", C.c_double), ("TriIneq", C.c_int),
+        ("adjust_TriIneq", C.c_int), ("PentIneq", C.c_int), ("HeptaIneq", C.c_int),
+        ("Pent_Trials", C.c_int), ("Hepta_Trials", C.c_int), ("include_Pent", C.c_int),
+        ("include_Hepta", C.c_int), ("root", C.c_int), ("use_diff", C.c_int),
+        ("time_limit", C.c_int), ("branchingStrategy", C.c_int),
+    ]
+
+
+DEFAULT_PARAMS = dict(init_bundle_iter=3, max_bundle_iter=15, triag_iter=5, pent_iter=5, hept_iter=5,
+                      max_outer_iter=20, extra_iter=10, violated_TriIneq=1e-3, TriIneq=5000,
+                      adjust_TriIneq=1, PentIneq=5000, HeptaIneq=5000, Pent_Trials=60,
+                      Hepta_Trials=50, include_Pent=1, include_Hepta=1, root=0, use_diff=1,
+                      time_limit=0, branchingStrategy=1)
+# values of the reference's shipped `params` file
+PARAMS_FILE = dict(DEFAULT_PARAMS, init_bundle_iter=5, violated_TriIneq=0.05)
+
+
+def make_params(**kw):
+    d = dict(PARAMS_FILE)
+    d.update(kw)
+    p = BiqBinParameters()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+class NodeState(C.Structure):
+    _fields_ = [(f, C.c_int) for f in (
+        "n", "status", "active", "prune", "giveup", "done", "count", "bdl_iter", "NIneq",
+        "NPentIneq", "NHeptaIneq", "bundle", "sep_tri", "sep_pent", "sep_hept", "heur_idx",
+        "improved", "ipm_calls", "ipm_iters", "max_cuts", "max_bundle_seen", "is_root", "serious",
+        "cuts_changed", "off_pent", "off_hept", "pad0")] + [(f, C.c_double) for f in (
+        "fixedvalue", "f", "f_test", "t", "bound", "basic_bound", "oldf", "LB", "diff", "viol3",
+        "viol5", "gap", "del_", "lmax")] + [("seed", C.c_uint64)]
+
+
+BUF = dict(state=0, L=1, L0=2, X=3, Xt=4, tri=5, tri_y=6, pent=7, pent_y=8, hept=9, hept_y=10,
+           gamma=11, gtest=12, g=13, G=14, F=15, lam=16, tri_list=17, tri_lval=18, kg_list=19,
+           kg_lval=20, fracsol=21, xbest=22, xheur=23, mask=24, phi=25, Xb=26, nvec=27, dgamma=28,
+           sa_val=29)
+PH = dict(SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OUTER_A=5, OUTER_B=6,
+          SEP_PENT=7, SEP_HEPT=8, OUTER_C=9, REBUILD=10, FINISH=11, IPM_BASIC=100, IPM_L0=101,
+          HEUR=102, TRISEP=103, PENT=104, HEPT=105, CSR=106, OPBT=107, QP=108, STEP=109)
+
+
+def _declare_ctx(L):
+    vp = C.c_void_p
+    L.bqb_ctx_create.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.bqb_ctx_create.restype = vp
+    L.bqb_ctx_destroy.argtypes = [vp]
+    L.bqb_ctx_bytes.argtypes = [vp]
+    L.bqb_ctx_bytes.restype = C.c_size_t
+    L.bqb_ctx_ld.argtypes = [vp]
+    L.bqb_ctx_ld.restype = C.c_int
+    L.bqb_ctx_last_ms.argtypes = [vp]
+    L.bqb_ctx_last_ms.restype = C.c_double
+    u64p = C.POINTER(C.c_uint64)
+    L.bqb_bound_batch.argtypes = [vp, C.c_int, N.iptr, N.iptr, N.dptr, N.dptr, N.iptr, u64p, N.dptr, N.dptr,
+                                  N.iptr, N.dptr, N.iptr, N.dptr, N.iptr, N.iptr]
+    L.bqb_ctx_load.argtypes = [vp, C.c_int, N.iptr, N.iptr, N.dptr, N.dptr, N.iptr, u64p]
+    L.bqb_ctx_run.argtypes = [vp, C.c_int, C.c_int]
+    L.bqb_ctx_bound.argtypes = [vp]
+    L.bqb_ctx_results.argtypes = [vp, N.dptr, N.dptr, N.iptr, N.dptr, N.iptr, N.dptr, N.iptr, N.iptr]
+    L.bqb_ctx_download.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+    L.bqb_ctx_upload.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+    L.bqb_ctx_sync.argtypes = [vp]
+    L.bqb_ctx_profile_ipm.argtypes = [vp, C.c_int]
+    L.bqb_ctx_profile_ipm.restype = None
+    L.bqb_ctx_ipm_ms.argtypes = [vp]
+    L.bqb_ctx_ipm_ms.restype = C.c_double
+
+
+class BoundContext:
+    """Device context for bounding batches of B&B nodes of one instance."""
+
+    def __init__(self, SP_L, params=None, max_nodes=1, max_bundle=0, max_tri=0, max_pent=0, max_hept=0):
+        L = N.lib()
+        self.SP_L = np.asfortranarray(SP_L, dtype=np.float64)
+        self.n0 = self.SP_L.shape[0]
+        self.params = params if params is not None else make_params()
+        self.h = L.bqb_ctx_create(self.n0, _d(self.SP_L), C.byref(self.params), max_nodes, max_bundle,
+                                  max_tri, max_pent, max_hept)
+        if not self.h:
+            raise RuntimeError("bqb_ctx_create: " + L.bqb_last_error().decode())
+        self.ld = L.bqb_ctx_ld(self.h)
+        self.max_nodes = max_nodes
+
+    def close(self):
+        if self.h:
+            N.lib().bqb_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _arrs(nodes, n0, xfixed, solX, LB, diff, is_root, seeds):
+        B = n0 - 1
+        xf = np.ascontiguousarray(np.reshape(xfixed, (nodes, B)), dtype=np.int32)
+        sx = np.ascontiguousarray(np.reshape(solX, (nodes, B)), dtype=np.int32)
+        lb = np.ascontiguousarray(np.broadcast_to(LB, (nodes,)), dtype=np.float64)
+        df = np.ascontiguousarray(np.broadcast_to(diff, (nodes,)), dtype=np.float64)
+        rt = np.ascontiguousarray(np.broadcast_to(is_root, (nodes,)), dtype=np.int32)
+        sd = np.ascontiguousarray(np.broadcast_to(seeds, (nodes,)), dtype=np.uint64)
+        return xf, sx, lb, df, rt, sd
+
+    def bound(self, xfixed, solX, LB=0.0, diff=0.0, is_root=0, seeds=0):
+        nodes = int(np.shape(xfixed)[0])
+        xf, sx, lb, df, rt, sd = self._arrs(nodes, self.n0, xfixed, solX, LB, diff, is_root, seeds)
+        B = self.n0 - 1
+        out = dict(bound=np.zeros(nodes), fracsol=np.zeros((nodes, B)), xbest=np.zeros((nodes, B), np.int32),
+                   LB=np.zeros(nodes), improved=np.zeros(nodes, np.int32), diff=np.zeros(nodes),
+                   status=np.zeros(nodes, np.int32), stats=np.zeros((nodes, 8), np.int32))
+        N.check(N.lib().bqb_bound_batch(
+            self.h, nodes, _i(xf), _i(sx), _d(lb), _d(df), _i(rt), sd.ctypes.data_as(C.POINTER(C.c_uint64)),
+            _d(out["bound"]), _d(out["fracsol"]), _i(out["xbest"]), _d(out["LB"]), _i(out["improved"]),
+            _d(out["diff"]), _i(out["status"]), _i(out["stats"])))
+        out["ms"] = N.lib().bqb_ctx_last_ms(self.h)
+        return out
+
+    def load(self, xfixed, solX, LB=0.0, diff=0.0, is_root=0, seeds=0):
+        nodes = int(np.shape(xfixed)[0])
+        xf, sx, lb, df, rt, sd = self._arrs(nodes, self.n0, xfixed, solX, LB, diff, is_root, seeds)
+        N.check(N.lib().bqb_ctx_load(self.h, nodes, _i(xf), _i(sx), _d(lb), _d(df), _i(rt),
+                                     sd.ctypes.data_as(C.POINTER(C.c_uint64))))
+
+    def run(self, phase, arg=0):
+        N.check(N.lib().bqb_ctx_run(self.h, PH[phase] if isinstance(phase, str) else phase, arg))
+        N.check(N.lib().bqb_ctx_sync(self.h))
+
+    def get(self, buf, node, dtype, count):
+        a = np.zeros(count, dtype=dtype)
+        N.check(N.lib().bqb_ctx_download(self.h, BUF[buf], node, a.ctypes.data, a.nbytes))
+        return a
+
+    def put(self, buf, node, arr):
+        a = np.ascontiguousarray(arr)
+        N.check(N.lib().bqb_ctx_upload(self.h, BUF[buf], node, a.ctypes.data, a.nbytes))
+
+    def state(self, node):
+        s = NodeState()
+        N.check(N.lib().bqb_ctx_download(self.h, 0, node, C.addressof(s), C.sizeof(s)))
+        return s
+
+    def set_state(self, node, s):
+        N.check(N.lib().bqb_ctx_upload(self.h, 0, node, C.addressof(s), C.sizeof(s)))
+
+    def matrix(self, buf, node, n):
+        a = self.get(buf, node, np.float64, self.ld * self.ld).reshape((self.ld, self.ld), order="F")
+        return np.array(a[:n, :n], order="F")

@@ -1,0 +1,407 @@
+// bqb_bundle.hip -- proximal bundle method over the cut multipliers of a
+// batch of nodes (reference src/bundle.c:9-373, 390-653).
+//   bundle_qp_kernel:   zeta = -F - G'gamma, Q = t G'G, then up to 50 rounds of
+//                       { c = zeta - t G'eta; lambda = argmin of the simplex QP;
+//                         eta = max(0, -gamma/t + G lambda);
+//                         dgamma = t (eta - G lambda) }  and gamma_test.
+//   bundle_step_kernel: after fct_eval(gamma_test) -- the serious / null step
+//                       test, X = X_bundle lambda, purge and append.
+// The simplex QP (k <= 64 bundle elements) is solved by the reference's
+// primal-dual interior-point loop on one wave with LDS-resident data; its
+// (k+1) x (k+1) KKT system [-Q-Diag(z./lambda) e; e' 0] is solved through the
+// Schur complement with a Cholesky factor of Q + Diag(z./lambda).
+#include <hip/hip_runtime.h>
+
+#include "bqb_batch.h"
+#include "bqb_cutops.h"
+#include "bqb_kernels.h"
+
+namespace bqb {
+
+constexpr int BT = 256;
+constexpr int KMAX = 64;
+
+__device__ __forceinline__ double wsum(double v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wmax(double v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wmin(double v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double bsum(double v, double *red)
+{
+    v = wsum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < BT / 64; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// Cholesky L L' = A of a k x k SPD matrix in LDS (column-major, ld k), one
+// wave, lane i owns row i.  Returns false if a pivot is not positive.
+__device__ bool wave_chol(double *A, int k, int lane)
+{
+    for (int j = 0; j < k; ++j) {
+        const double d = A[j + j * k];
+        if (!(d > 0.0)) return false;
+        const double ljj = sqrt(d);
+        if (lane > j && lane < k) A[lane + j * k] = A[lane + j * k] / ljj;
+        __builtin_amdgcn_wave_barrier();
+        if (lane == j) A[j + j * k] = ljj;
+        __builtin_amdgcn_wave_barrier();
+        if (lane > j && lane < k) {
+            const double lij = A[lane + j * k];
+            for (int l = j + 1; l <= lane; ++l) A[lane + l * k] -= lij * A[l + j * k];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    return true;
+}
+
+// Solve (L L') x = b in place (b in LDS), one wave.
+__device__ void wave_cholsolve(const double *Lc, int k, double *b, int lane)
+{
+    for (int j = 0; j < k; ++j) {
+        if (lane == j) b[j] = b[j] / Lc[j + j * k];
+        __builtin_amdgcn_wave_barrier();
+        const double xj = b[j];
+        if (lane > j && lane < k) b[lane] -= Lc[lane + j * k] * xj;
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int j = k - 1; j >= 0; --j) {
+        if (lane == j) b[j] = b[j] / Lc[j + j * k];
+        __builtin_amdgcn_wave_barrier();
+        const double xj = b[j];
+        if (lane < j) b[lane] -= Lc[j + lane * k] * xj;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// bundle.c:390-653 on one wave.  Q (k x k), c, lambda in LDS.
+__device__ int solve_lambda_wave(int k, const double *Q, const double *c, double *lam, double *A,
+                                 double *u, double *w, int lane)
+{
+    const bool in = lane < k;
+    double l = in ? 1.0 / k : 0.0;
+    if (in) lam[lane] = l;
+    __builtin_amdgcn_wave_barrier();
+    auto qlam = [&](double lv_unused) {
+        (void)lv_unused;
+        double s = 0.0;
+        if (in)
+            for (int j = 0; j < k; ++j) s += Q[lane + j * k] * lam[j];
+        return s;
+    };
+    double Ql = qlam(0.0);
+    const double ci = in ? c[lane] : 0.0;
+    const double tmpv = in ? Ql + ci : 1e300;
+    const double mintmp = wmin(tmpv);
+    double y, z;
+    if (mintmp > 1) { y = 0.0; z = in ? tmpv : 0.0; }
+    else { y = mintmp - 1.0; z = in ? tmpv - y : 0.0; }
+    double mu = wsum(in ? z * l : 0.0) / k * 0.5;
+    double res_p = 1.0 - wsum(in ? l : 0.0);
+    double temp = wsum(in ? l * Ql : 0.0);
+    double D_cost = y - 0.5 * temp;
+    double P_cost = wsum(in ? l * ci : 0.0) + 0.5 * temp;
+    double gap = P_cost - D_cost;
+    int cnt = 0;
+    while (fabs(gap) > 1e-5) {
+        ++cnt;
+        for (int j = 0; j < k; ++j)
+            if (in) A[lane + j * k] = Q[lane + j * k];
+        __builtin_amdgcn_wave_barrier();
+        if (in) {
+            A[lane + lane * k] = -(-Q[lane + lane * k] - z / l);
+            w[lane] = ci - y + Ql;
+            u[lane] = 1.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (!wave_chol(A, k, lane)) return ERR_QP;
+        wave_cholsolve(A, k, u, lane);
+        wave_cholsolve(A, k, w, lane);
+        const double ui = in ? u[lane] : 0.0, wi = in ? w[lane] : 0.0;
+        const double su = wsum(ui), sw = wsum(wi);
+        const double dy = (res_p + sw) / su;
+        const double dl = in ? ui * dy - wi : 0.0;
+        const double dz = in ? 1.0 / l * (mu - l * z - z * dl) : 0.0;
+        double ap = wmax(in ? -dl / l : -1e9);
+        double ad = wmax(in ? -dz / z : -1e9);
+        ap = (ap > 0) ? ((0.99 / ap < 1.0) ? 0.99 / ap : 1.0) : 1.0;
+        ad = (ad > 0) ? ((0.99 / ad < 1.0) ? 0.99 / ad : 1.0) : 1.0;
+        y += ad * dy;
+        if (in) { l = l + ap * dl; z = z + ad * dz; lam[lane] = l; }
+        __builtin_amdgcn_wave_barrier();
+        res_p = 1 - wsum(in ? l : 0.0);
+        Ql = qlam(0.0);
+        mu = wsum(in ? l * z : 0.0) / k * 0.4;
+        if (ap + ad > 1.8) mu *= 0.2;
+        temp = wsum(in ? l * Ql : 0.0);
+        D_cost = y - 0.5 * temp;
+        P_cost = wsum(in ? l * ci : 0.0) + 0.5 * temp;
+        gap = P_cost - D_cost;
+        if (cnt > 30) break;
+    }
+    return 0;
+}
+
+__global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
+{
+    __shared__ double s_Q[KMAX * KMAX];
+    __shared__ double s_A[KMAX * KMAX];
+    __shared__ double s_zeta[KMAX], s_c[KMAX], s_lam[KMAX], s_u[KMAX], s_w[KMAX];
+    __shared__ double s_red[BT / 64];
+    __shared__ int s_done, s_stat;
+    __shared__ double s_t;
+
+    const int node = blockIdx.x;
+    if (node >= d.nodes || !p.mask[node]) return;
+    NodeState *sp = p.st + node;
+    const NodeState s = *sp;
+    const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
+    const int k = s.bundle;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (k > KMAX || k < 1) {
+        if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
+        return;
+    }
+    const double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
+    const double *F = p.F + (size_t)node * d.max_bundle;
+    const double *gam = p.gamma + (size_t)node * d.max_m;
+    double *eta = p.eta + (size_t)node * d.max_m;
+    double *dg = p.dgamma + (size_t)node * d.max_m;
+    double *gt = p.gtest + (size_t)node * d.max_m;
+    double *tmp = p.tmpm + (size_t)node * d.max_m;
+    double t = s.t;
+
+    // zeta = -F - G' gamma  (bundle.c:57-63)
+    for (int j = wave; j < k; j += BT / 64) {
+        double acc = 0.0;
+        for (int l = lane; l < m; l += 64) acc += G[l + (size_t)j * m] * gam[l];
+        acc = wsum(acc);
+        // with no cuts dgemv returns early and leaves zeta = F
+        if (lane == 0) s_zeta[j] = (m > 0) ? -F[j] + (-1.0) * acc : F[j];
+    }
+    // Q = t G'G  (bundle.c:309-315)
+    for (int q = wave; q < k * k; q += BT / 64) {
+        const int i = q % k, j = q / k;
+        if (i < j) continue;
+        double acc = 0.0;
+        for (int l = lane; l < m; l += 64) acc += G[l + (size_t)i * m] * G[l + (size_t)j * m];
+        acc = wsum(acc);
+        if (lane == 0) s_Q[i + j * k] = s_Q[j + i * k] = (m > 0) ? t * acc : 0.0;
+    }
+    for (int l = threadIdx.x; l < m; l += BT) eta[l] = 0.0;
+    if (threadIdx.x == 0) { s_done = 0; s_stat = 0; }
+    __syncthreads();
+
+    double dir_prev = 0.0;
+    int cnt = 0;
+    while (!s_done) {
+        ++cnt;
+        // c = zeta - t G' eta
+        for (int j = wave; j < k; j += BT / 64) {
+            double acc = 0.0;
+            for (int l = lane; l < m; l += 64) acc += G[l + (size_t)j * m] * eta[l];
+            acc = wsum(acc);
+            if (lane == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * acc : s_zeta[j];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const int st = solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
+            if (lane == 0 && st) s_stat = st;
+        }
+        __syncthreads();
+        if (s_stat) break;
+        // tmp = G lambda; eta = max(0, -gamma/t + tmp); dgamma = t (eta - tmp)
+        double nrm = 0.0;
+        for (int l = threadIdx.x; l < m; l += BT) {
+            double acc = 0.0;
+            for (int j = 0; j < k; ++j) acc += G[l + (size_t)j * m] * s_lam[j];
+            tmp[l] = acc;
+            double e = -gam[l] / t + acc;
+            e = (e > 0) ? e : 0;
+            eta[l] = e;
+            const double dgl = t * (e - acc);
+            dg[l] = dgl;
+            nrm += dgl * dgl;
+        }
+        nrm = bsum(nrm, s_red);
+        const double dir = sqrt(nrm);
+        if (threadIdx.x == 0) {
+            int done = 0;
+            if (fabs(dir - dir_prev) / (1.0 + dir) < 1e-5) done = 1;
+            if (cnt >= 50) { done = 1; t *= 0.95; }
+            s_done = done;
+            s_t = t;
+        }
+        dir_prev = dir;
+        __syncthreads();
+        t = s_t;
+    }
+    if (s_stat) {
+        if (threadIdx.x == 0) { sp->status = s_stat; sp->active = 0; p.mask[node] = 0; }
+        return;
+    }
+    for (int l = threadIdx.x; l < m; l += BT) gt[l] = gam[l] + dg[l];
+    double *lam = p.lambda + (size_t)node * d.max_bundle;
+    for (int j = threadIdx.x; j < k; j += BT) lam[j] = s_lam[j];
+    if (threadIdx.x == 0) sp->t = t;
+}
+
+// Column / slab copy inside one node's bundle (memmove order preserved by
+// the caller: destinations precede sources).
+__device__ void bundle_move(const BatchDims &d, const BatchPtrs &p, int node, int m, int from, int to)
+{
+    const size_t slab = (size_t)d.ld * d.ld;
+    double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
+    double *Xb = p.Xb + (size_t)node * d.max_bundle * slab;
+    for (int l = threadIdx.x; l < m; l += BT) G[(size_t)to * m + l] = G[(size_t)from * m + l];
+    for (size_t e = threadIdx.x; e < slab; e += BT) Xb[to * slab + e] = Xb[from * slab + e];
+    if (threadIdx.x == 0) p.F[(size_t)node * d.max_bundle + to] = p.F[(size_t)node * d.max_bundle + from];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs p)
+{
+    __shared__ double s_red[BT / 64];
+    __shared__ double s_lam[KMAX];
+    __shared__ int s_keep[KMAX];
+    const int node = blockIdx.x;
+    if (node >= d.nodes || !p.mask[node]) return;
+    NodeState *sp = p.st + node;
+    const NodeState s = *sp;
+    const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
+    const int k = s.bundle;
+    const size_t slab = (size_t)d.ld * d.ld;
+    double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
+    double *F = p.F + (size_t)node * d.max_bundle;
+    const double *lam = p.lambda + (size_t)node * d.max_bundle;
+    double *gam = p.gamma + (size_t)node * d.max_m;
+    const double *gt = p.gtest + (size_t)node * d.max_m;
+    double *g = p.g + (size_t)node * d.max_m;
+    const double *Xt = p.Xt + node * slab;
+    const int ipm_st = p.ipm_status[node];
+
+    for (int j = threadIdx.x; j < k; j += BT) s_lam[j] = lam[j];
+    // f_test = phi + sum(gamma_test), g = 1 - B(X_test)  (bundle.c:259-270)
+    double sg = 0.0;
+    for (int l = threadIdx.x; l < m; l += BT) sg += gt[l];
+    sg = bsum(sg, s_red);
+    const double f_test = p.phi[node] + (m > 0 ? sg : 0.0);
+    {
+        const TriCut *tri = p.tri + (size_t)node * d.max_tri;
+        const KgonCut *pent = p.pent + (size_t)node * d.max_pent;
+        const KgonCut *hept = p.hept + (size_t)node * d.max_hept;
+        for (int c = threadIdx.x; c < m; c += BT)
+            g[c] = 1.0 - cut_B(Xt, d.ld, c, s.NIneq, s.NPentIneq, tri, pent, hept);
+    }
+    // f_appr = <F + G' gamma_test, lambda>  (bundle.c:76-83)
+    double fa = 0.0;
+    for (int j = threadIdx.x >> 6; j < k; j += BT / 64) {
+        double acc = 0.0;
+        for (int l = threadIdx.x & 63; l < m; l += 64) acc += G[l + (size_t)j * m] * gt[l];
+        acc = wsum(acc);
+        if ((threadIdx.x & 63) == 0) fa += (F[j] + acc) * s_lam[j];
+    }
+    fa = bsum(fa, s_red);
+    const double del = s.f - fa;
+    double lmax = -1e9;
+    for (int i = 0; i < k; ++i) lmax = (s_lam[i] > lmax) ? s_lam[i] : lmax;
+    const int serious = (s.f - f_test > 0.05 * del) ? 1 : 0;
+    if (ipm_st) {
+        if (threadIdx.x == 0) { sp->status = ipm_st; sp->active = 0; p.mask[node] = 0; }
+        return;
+    }
+    double t = s.t;
+    double f = s.f;
+    if (serious) {
+        for (int l = threadIdx.x; l < m; l += BT) gam[l] = gt[l];
+        f = f_test;
+        // X = X_bundle * lambda, each entry summed over the bundle in order
+        double *X = p.X + node * slab;
+        const double *Xb = p.Xb + (size_t)node * d.max_bundle * slab;
+        for (size_t e = threadIdx.x; e < slab; e += BT) {
+            double acc = 0.0;
+            for (int j = 0; j < k; ++j) acc += Xb[j * slab + e] * s_lam[j];
+            X[e] = acc;
+        }
+        t *= 1.01;
+    } else {
+        t /= 1.01;
+    }
+    __syncthreads();
+    // purge (all k on a serious step, the first k-1 on a null step)
+    const int kp = serious ? k : k - 1;
+    int next = 0, subtracted = 0;
+    for (int i = 0; i < kp; ++i) {
+        if (s_lam[i] < 0.01 * lmax) ++subtracted;
+        else {
+            if (i > next) bundle_move(d, p, node, m, i, next);
+            ++next;
+        }
+    }
+    int bundle = k - subtracted;
+    if (bundle >= d.max_bundle) {
+        if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
+        return;
+    }
+    int pos;
+    if (serious) pos = bundle;
+    else {
+        bundle_move(d, p, node, m, k - 1, bundle);
+        pos = bundle - 1;
+    }
+    for (int l = threadIdx.x; l < m; l += BT) G[(size_t)pos * m + l] = g[l];
+    double *Xdst = p.Xb + ((size_t)node * d.max_bundle + pos) * slab;
+    for (size_t e = threadIdx.x; e < slab; e += BT) Xdst[e] = Xt[e];
+    // F = <L, X_test> (bundle.c:152-163)
+    double acc = 0.0;
+    const double *L = p.L + node * slab;
+    for (int e = threadIdx.x; e < s.n * s.n; e += BT) {
+        const int i = e % s.n, j = e / s.n;
+        acc += L[i + (size_t)j * d.ld] * Xt[i + (size_t)j * d.ld];
+    }
+    acc = bsum(acc, s_red);
+    if (threadIdx.x == 0) {
+        F[pos] = acc;
+        ++bundle;
+        sp->bundle = bundle;
+        if (bundle > s.max_bundle_seen) sp->max_bundle_seen = bundle;
+        sp->t = t;
+        sp->f = f;
+        sp->f_test = f_test;
+        sp->del = del;
+        sp->lmax = lmax;
+        sp->serious = serious;
+        sp->ipm_calls = s.ipm_calls + 1;
+        sp->ipm_iters = s.ipm_iters + p.ipm_iters[node];
+    }
+    (void)s_keep;
+}
+
+}  // namespace bqb
+
+extern "C" hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, hipStream_t s)
+{
+    hipLaunchKernelGGL(bqb::bundle_qp_kernel, dim3(d->nodes), dim3(bqb::BT), 0, s, *d, *p);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bqb_launch_bundle_step(const BatchDims *d, const BatchPtrs *p, hipStream_t s)
+{
+    hipLaunchKernelGGL(bqb::bundle_step_kernel, dim3(d->nodes), dim3(bqb::BT), 0, s, *d, *p);
+    return hipGetLastError();
+}

@@ -1,0 +1,296 @@
+// bqb_heur.hip -- primal heuristic of a batch of nodes (reference
+// src/heuristic.c:19-317, src/bab_functions.c:96-134):
+//   Z = X; repeat { Z = chol_lower(Z); x = best of n0 random-hyperplane cuts
+//   sign(Z v), each improved by 1-opt; if it beats the best so far, restart
+//   from Z = (1-c) X + c xh xh' } until two rounds in a row do not improve;
+// then the node's incumbent is updated with the last round's best cut.
+// Random numbers come from the node's counter streams: hyperplane h of
+// round r draws v_j = U(key(heur key, r+1), h*1024 + j) - 1/2.
+// The Cholesky factor uses the workgroup-distributed tile; the hyperplane
+// products are formed per (hyperplane, row) in the reference's order without
+// fused multiply-add; 1-opt and the cut values run on integer-valued data
+// and are exact.
+#include <hip/hip_runtime.h>
+
+#include "bqb_batch.h"
+#include "bqb_dist.h"
+#include "bqb_kernels.h"
+#include "bqb_rng.h"
+
+namespace bqb {
+
+constexpr int HEUR_MAX_ROUNDS = 1000;
+constexpr int NMAX_GW = 256;      // vertices (n0) supported
+constexpr int GW_BATCH = 16;      // hyperplanes held in LDS at a time
+
+// argmax with the first index on ties across a wave
+__device__ __forceinline__ void wave_argmax(double &v, int &i)
+{
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(i, o, 64);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int P, int R>
+__global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
+{
+    constexpr int NT = P * P;
+    constexpr int NW = NT / 64;
+    constexpr int NMAX = P * R;
+    constexpr int PL = (NMAX + 63) / 64;
+    __shared__ double s_col[2 * NMAX];
+    __shared__ double s_xh[NMAX];
+    __shared__ signed char s_sign[GW_BATCH][NMAX];
+    __shared__ signed char s_lift[GW_BATCH][NMAX_GW];
+    __shared__ int s_sol[NMAX_GW];
+    __shared__ double s_val[GW_BATCH];
+    __shared__ int s_free[NMAX_GW];
+    __shared__ int s_best_sign[NMAX];
+    __shared__ double s_red[NW];
+    __shared__ double s_best;
+    __shared__ int s_flag;
+
+    const int node = blockIdx.x;
+    if (node >= d.nodes || !p.mask[node]) return;
+    NodeState *sp = p.st + node;
+    const NodeState s = *sp;
+    const Thr t = thr_id<P>();
+    const int n = s.n, ld = d.ld, n0 = d.n0, B = n0 - 1;
+    const size_t slab = (size_t)ld * ld;
+    const double *X = p.X + node * slab;
+    const double *L = p.L + node * slab;
+    double *Zs = p.Zh + node * slab;
+    const int *xf = p.xfixed + (size_t)node * B;
+    const int *sx = p.solX + (size_t)node * B;
+    int *xo = p.xheur + (size_t)node * B;
+    const uint64_t key = BQB_KEY_HEUR(s.seed, s.heur_idx);
+    const int lane = t.lane, wave = t.wave;
+    if (n > NMAX || n0 > NMAX_GW || n < 1) {
+        if (t.tid == 0) { sp->status = ERR_SIZE; sp->active = 0; p.mask[node] = 0; }
+        return;
+    }
+    if (t.tid == 0) {
+        int nf = 0;
+        for (int i = 0; i < B; ++i)
+            if (!xf[i]) s_free[nf++] = i;
+    }
+    // first random cut xh in {-1,1}^n (heuristic.c:38-39)
+    for (int i = t.tid; i < NMAX; i += NT)
+        s_xh[i] = (i < n) ? (double)(2 * (int)(bqb_draw(key, (uint64_t)i) >> 63) - 1) : 0.0;
+    __syncthreads();
+
+    // fh = value of its lift (heuristic.c:42-54)
+    if (t.tid == 0) {
+        int index = 0;
+        for (int i = 0; i < B; ++i) {
+            if (xf[i]) s_sol[i] = sx[i];
+            else { s_sol[i] = (int)((s_xh[index] + 1) / 2.0); ++index; }
+        }
+    }
+    __syncthreads();
+    double fh;
+    {
+        double acc = 0.0;
+        for (int e = t.tid; e < B * B; e += NT) {
+            const int i = e / B, j = e % B;
+            if (s_sol[i] && s_sol[j]) acc += p.SP_L[j + (size_t)i * n0];
+        }
+        fh = wg_sum<P>(acc, s_red, t);
+    }
+
+    double heur_val = -1e9;
+    int done = 0, round = 0, status = 0;
+    double A[R][R];
+    const Slab sX = make_slab<P>(X, ld, t);
+    const Slab sZ = make_slab<P>(Zs, ld, t);
+    dload<P, R>(A, sX);                               // Z = X
+    while (done < 2) {
+        ++done;
+        if (round >= HEUR_MAX_ROUNDS) break;
+        const uint64_t rkey = bqb_key(key, (uint64_t)round + 1);
+        ++round;
+        if (!chol_lower<P, R>(A, n, s_col, t)) { status = ERR_HEUR_CHOL; break; }
+        dstore<P, R>(A, sZ);
+        __syncthreads();
+
+        // ---- GW_heuristic (heuristic.c:119-186): n0 hyperplanes ----------
+        for (int h0 = 0; h0 < n0; h0 += GW_BATCH) {
+            const int hb = min(GW_BATCH, n0 - h0);
+            // sign(Z v) per (hyperplane, row), products summed in j order
+            for (int e = t.tid; e < hb * n; e += NT) {
+                const int hh = e / n, i = e % n;
+                const uint64_t base = (uint64_t)(h0 + hh) * BQB_GW_STRIDE;
+                double sca = 0.0;
+                for (int j = 0; j <= i; ++j) {
+                    const double v = bqb_u01(bqb_draw(rkey, base + j)) - 0.5;
+                    sca = __dadd_rn(sca, __dmul_rn(v, Zs[i + (size_t)j * ld]));
+                }
+                s_sign[hh][i] = (sca < 0) ? -1 : 1;
+            }
+            __syncthreads();
+            // 1-opt (heuristic.c:195-296), one wave per hyperplane: the cut
+            // stays in LDS, Lx in registers (row i = lane + 64 q)
+            for (int hh = wave; hh < hb; hh += NW) {
+                signed char *xcur = s_sign[hh];
+                double Lx[PL];
+#pragma unroll
+                for (int q = 0; q < PL; ++q) {
+                    const int i = lane + 64 * q;
+                    double acc = 0.0;
+                    if (i < n)
+                        for (int j = 0; j < n; ++j) acc += L[j + (size_t)i * ld] * (double)xcur[j];
+                    Lx[q] = acc;
+                }
+                double bv = -1e9;
+                int bi = 0x7fffffff;
+#pragma unroll
+                for (int q = 0; q < PL; ++q) {
+                    const int i = lane + 64 * q;
+                    if (i < n) {
+                        const double dl = L[i + (size_t)i * ld] - xcur[i] * Lx[q];
+                        if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
+                    }
+                }
+                wave_argmax(bv, bi);
+                int flips = 0;
+                while (bv > 0.001 && flips < 100 * n + 100) {
+                    ++flips;
+                    const int index = bi;
+                    const int xi = xcur[index];
+#pragma unroll
+                    for (int q = 0; q < PL; ++q) {
+                        const int i = lane + 64 * q;
+                        if (i < n) {
+                            const double lij = L[index + (size_t)i * ld];
+                            if (fabs(lij) > 0.001) Lx[q] = (xi > 0) ? Lx[q] - 2 * lij : Lx[q] + 2 * lij;
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == (index & 63)) xcur[index] = (signed char)(-xi);
+                    __builtin_amdgcn_wave_barrier();
+                    bv = -1e9;
+                    bi = 0x7fffffff;
+#pragma unroll
+                    for (int q = 0; q < PL; ++q) {
+                        const int i = lane + 64 * q;
+                        if (i < n) {
+                            const double dl = L[i + (size_t)i * ld] - xcur[i] * Lx[q];
+                            if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
+                        }
+                    }
+                    wave_argmax(bv, bi);
+                }
+                // lift to the original vertices (fixed ones keep their value)
+                for (int i = lane; i < B; i += 64) s_lift[hh][i] = xf[i] ? (signed char)sx[i] : (signed char)0;
+                __builtin_amdgcn_wave_barrier();
+                for (int a = lane; a < n - 1; a += 64) s_lift[hh][s_free[a]] = (signed char)((xcur[a] + 1) / 2);
+                __builtin_amdgcn_wave_barrier();
+                // its cut value (update_best -> evaluateSolution)
+                double acc = 0.0;
+                for (int e = lane; e < B * B; e += 64) {
+                    const int i = e / B, j = e % B;
+                    if (s_lift[hh][i] && s_lift[hh][j]) acc += p.SP_L[j + (size_t)i * n0];
+                }
+                acc = wave_sum(acc);
+                if (lane == 0) s_val[hh] = acc;
+            }
+            __syncthreads();
+            // update_best: first strictly better hyperplane in order
+            if (t.tid == 0) {
+                double bb = (h0 == 0) ? -1e9 : s_best;
+                int bh = -1;
+                for (int hh = 0; hh < hb; ++hh)
+                    if (bb < s_val[hh]) { bb = s_val[hh]; bh = hh; }
+                s_best = bb;
+                s_flag = bh;
+            }
+            __syncthreads();
+            if (s_flag >= 0)
+                for (int i = t.tid; i < n; i += NT) s_best_sign[i] = s_sign[s_flag][i];
+            __syncthreads();
+        }
+        heur_val = s_best;
+        // x = lift of this round's best cut
+        if (t.tid == 0) {
+            int index = 0;
+            for (int i = 0; i < B; ++i) {
+                if (xf[i]) xo[i] = sx[i];
+                else { xo[i] = (s_best_sign[index] + 1) / 2; ++index; }
+            }
+        }
+        __syncthreads();
+        if (heur_val > fh) {
+            done = 0;
+            fh = heur_val;
+            if (t.tid == 0) {
+                int index = 0;
+                for (int i = 0; i < B; ++i)
+                    if (!xf[i]) { s_xh[index] = 2 * xo[i] - 1; ++index; }
+                s_xh[n - 1] = -1.0;
+            }
+            __syncthreads();
+        }
+        const double c = 0.3 + 0.6 * bqb_u01(bqb_draw(rkey, (uint64_t)n0 * BQB_GW_STRIDE));
+        const double alpha = 1.0 - c;
+        // Z = (1-c) X, lower triangle += c xh xh' (dscal + dsyr 'L')
+        dload<P, R>(A, sX);
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+            for (int b = 0; b < R; ++b) {
+                const int I = t.tr + P * a, J = t.tc + P * b;
+                double z = alpha * A[a][b];
+                if (I >= J && I < n) z = z + c * s_xh[J] * s_xh[I];
+                A[a][b] = z;
+            }
+    }
+
+    // updateSolution (bab_functions.c:114-134) against the node's incumbent
+    __syncthreads();
+    if (status) {
+        if (t.tid == 0) { sp->status = status; sp->active = 0; p.mask[node] = 0; }
+        return;
+    }
+    double acc = 0.0;
+    for (int e = t.tid; e < B * B; e += NT) {
+        const int i = e / B, j = e % B;
+        if (xo[i] && xo[j]) acc += p.SP_L[j + (size_t)i * n0];
+    }
+    acc = wg_sum<P>(acc, s_red, t);
+    if (acc > s.LB) {
+        int *xb = p.xbest + (size_t)node * B;
+        for (int i = t.tid; i < B; i += NT) xb[i] = xo[i];
+        if (t.tid == 0) { sp->LB = acc; sp->improved = 1; }
+    }
+}
+
+template <int P, int R>
+static hipError_t launch_heur(const BatchDims &d, const BatchPtrs &p, hipStream_t s)
+{
+    hipLaunchKernelGGL((heur_kernel<P, R>), dim3(d.nodes), dim3(P * P), 0, s, d, p);
+    return hipGetLastError();
+}
+
+}  // namespace bqb
+
+extern "C" hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, hipStream_t s)
+{
+    using namespace bqb;
+    switch (tile) {
+    case 1604: return launch_heur<16, 4>(*d, *p, s);
+    case 1605: return launch_heur<16, 5>(*d, *p, s);
+    case 1606: return launch_heur<16, 6>(*d, *p, s);
+    case 1607: return launch_heur<16, 7>(*d, *p, s);
+    case 3204: return launch_heur<32, 4>(*d, *p, s);
+    case 3205: return launch_heur<32, 5>(*d, *p, s);
+    default: return hipErrorInvalidValue;
+    }
+}

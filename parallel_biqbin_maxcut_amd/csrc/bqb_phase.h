@@ -1,0 +1,18 @@
+/* bqb_phase.h -- phases of the batched SDPbound control kernel. */
+#ifndef BQB_PHASE_H
+#define BQB_PHASE_H
+enum {
+    PH_SETUP = 0,        /* createSubproblem, getFixedValue, node state */
+    PH_POST_BASIC = 1,   /* f, basic bound, fracsol after the basic IPM */
+    PH_CHECK_BASIC = 2,  /* prune / give-up after the first heuristic */
+    PH_BUNDLE_INIT = 3,  /* bundle initialisation after the first separation */
+    PH_OUTER_BEGIN = 4,  /* ++count, oldf */
+    PH_OUTER_A = 5,      /* bound and prune after the bundle method */
+    PH_OUTER_B = 6,      /* prune / give-up after the heuristic */
+    PH_SEP_PENT = 7,     /* pentagonal separation decision */
+    PH_SEP_HEPT = 8,     /* heptagonal separation decision */
+    PH_OUTER_C = 9,      /* stop test, fracsol, gamma from the cuts */
+    PH_REBUILD = 10,     /* extend the bundle after fct_eval(gamma) */
+    PH_FINISH = 11
+};
+#endif

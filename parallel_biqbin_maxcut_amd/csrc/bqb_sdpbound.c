@@ -1,0 +1,430 @@
+/*
+ * bqb_sdpbound.c -- host driver of the batched node bound (C).
+ *
+ * A context owns the device arenas of up to max_nodes node slots for one
+ * instance (SP->L, params).  bqb_bound_batch() bounds a batch of B&B nodes
+ * with the reference's SDPbound algorithm (src/bounding.c:24-342): every
+ * per-node decision is taken on the device by the control kernel; the host
+ * only sequences the phases, which are the same for all nodes of the batch
+ * (nodes that have finished simply mask themselves out), and polls how many
+ * nodes are still active once per outer iteration.
+ *
+ * Each node's bound is a function of (node, LB at batch start, diff, seed):
+ * the incumbent a node reads is the batch-start LB raised by its own
+ * heuristic solutions (with one node per batch this is exactly the
+ * reference's Bab_LBGet()).
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "bqb_batch.h"
+#include "bqb_kernels.h"
+#include "bqb_phase.h"
+#include "bqb_runtime.h"
+
+hipError_t bqb_launch_control(const BatchDims *d, const BatchPtrs *p, int phase, int arg, hipStream_t s);
+hipError_t bqb_launch_csr_build(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
+hipError_t bqb_launch_opbt(const BatchDims *d, const BatchPtrs *p, int use_gtest, hipStream_t s);
+hipError_t bqb_launch_trisep(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
+hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStream_t s);
+hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, hipStream_t s);
+hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
+hipError_t bqb_launch_bundle_step(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
+
+struct bqb_ctx {
+    BatchDims d;
+    BatchPtrs p;
+    BqbParams params;
+    int max_nodes;
+    int tile;
+    void *arena;
+    size_t arena_bytes;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    NodeState *h_st;
+    double last_ms;
+    double ipm_ms;        /* accumulated time of the IPM launches (events) */
+    long long ipm_iters;  /* sum of IPM iterations over all IPM calls */
+    long long ipm_flops;  /* algorithmic IPM flops of the last batch */
+    int profile_ipm;
+};
+
+#define CK(call, what) do { if (bqb_check((call), (what))) goto fail; } while (0)
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int max_nodes,
+                        int max_bundle, int max_tri, int max_pent, int max_hept)
+{
+    if (bqb_ensure_init()) return NULL;
+    if (n0 < 2 || max_nodes < 1) { bqb_set_error("bad context size"); return NULL; }
+    if (max_bundle <= 0) max_bundle = 64;
+    if (max_bundle > 64) { bqb_set_error("max_bundle > 64 is not supported"); return NULL; }
+    if (max_tri <= 0) max_tri = 8192;
+    if (max_pent <= 0) max_pent = 2048;
+    if (max_hept <= 0) max_hept = 2048;
+    int tile = bqb_ipm_tile_for(n0);
+    if (!tile) { bqb_set_error("n0 = %d exceeds the largest compiled tile", n0); return NULL; }
+    bqb_ctx *c = calloc(1, sizeof(bqb_ctx));
+    if (!c) { bqb_set_error("calloc"); return NULL; }
+    c->params = *params;
+    if (c->params.adjust_TriIneq) c->params.TriIneq = n0 * 10;   /* process_input.c:135-136 */
+    if (c->params.TriIneq > BQB_TRI_LIST_MAX || c->params.TriIneq < 1) {
+        bqb_set_error("TriIneq = %d outside 1..%d", c->params.TriIneq, BQB_TRI_LIST_MAX);
+        free(c);
+        return NULL;
+    }
+    if (c->params.Pent_Trials * 3 > BQB_SA_CHAINS || c->params.Hepta_Trials * 4 > BQB_SA_CHAINS) {
+        bqb_set_error("Pent_Trials*3 and Hepta_Trials*4 must not exceed %d", BQB_SA_CHAINS);
+        free(c);
+        return NULL;
+    }
+    c->max_nodes = max_nodes;
+    c->tile = tile;
+    BatchDims *d = &c->d;
+    d->nodes = max_nodes;
+    d->n0 = n0;
+    d->ld = (tile / 100) * (tile % 100);
+    d->max_bundle = max_bundle;
+    d->max_tri = max_tri;
+    d->max_pent = max_pent;
+    d->max_hept = max_hept;
+    d->max_m = max_tri + max_pent + max_hept;
+    d->tri_list = c->params.TriIneq;
+    d->kg_list = BQB_SA_CHAINS;
+    d->max_contrib = 3 * max_tri + 10 * max_pent + 21 * max_hept;
+    d->npairs = d->ld * (d->ld - 1) / 2;
+
+    const size_t N = (size_t)max_nodes, slab = (size_t)d->ld * d->ld, B = (size_t)(n0 - 1);
+    struct { void **ptr; size_t bytes; } items[] = {
+        {(void **)&c->p.st, sizeof(NodeState) * N},
+        {(void **)&c->p.params, sizeof(BqbParams)},
+        {(void **)&c->p.SP_L, sizeof(double) * (size_t)n0 * n0},
+        {(void **)&c->p.xfixed, sizeof(int) * N * B},
+        {(void **)&c->p.solX, sizeof(int) * N * B},
+        {(void **)&c->p.nvec, sizeof(int) * N},
+        {(void **)&c->p.mask, sizeof(int) * N},
+        {(void **)&c->p.L, sizeof(double) * N * slab},
+        {(void **)&c->p.L0, sizeof(double) * N * slab},
+        {(void **)&c->p.X, sizeof(double) * N * slab},
+        {(void **)&c->p.Xt, sizeof(double) * N * slab},
+        {(void **)&c->p.Zh, sizeof(double) * N * slab},
+        {(void **)&c->p.ipm_work, sizeof(double) * N * 3 * slab},
+        {(void **)&c->p.phi, sizeof(double) * N},
+        {(void **)&c->p.ipm_iters, sizeof(int) * N},
+        {(void **)&c->p.ipm_trials, sizeof(int) * N},
+        {(void **)&c->p.ipm_status, sizeof(int) * N},
+        {(void **)&c->p.Xb, sizeof(double) * N * max_bundle * slab},
+        {(void **)&c->p.G, sizeof(double) * N * max_bundle * d->max_m},
+        {(void **)&c->p.F, sizeof(double) * N * max_bundle},
+        {(void **)&c->p.lambda, sizeof(double) * N * max_bundle},
+        {(void **)&c->p.zeta, sizeof(double) * N * max_bundle},
+        {(void **)&c->p.gamma, sizeof(double) * N * d->max_m},
+        {(void **)&c->p.dgamma, sizeof(double) * N * d->max_m},
+        {(void **)&c->p.gtest, sizeof(double) * N * d->max_m},
+        {(void **)&c->p.eta, sizeof(double) * N * d->max_m},
+        {(void **)&c->p.g, sizeof(double) * N * d->max_m},
+        {(void **)&c->p.tmpm, sizeof(double) * N * d->max_m},
+        {(void **)&c->p.tri, sizeof(TriCut) * N * max_tri},
+        {(void **)&c->p.pent, sizeof(KgonCut) * N * max_pent},
+        {(void **)&c->p.hept, sizeof(KgonCut) * N * max_hept},
+        {(void **)&c->p.tri_y, sizeof(double) * N * max_tri},
+        {(void **)&c->p.pent_y, sizeof(double) * N * max_pent},
+        {(void **)&c->p.hept_y, sizeof(double) * N * max_hept},
+        {(void **)&c->p.tri_lst, sizeof(TriCut) * N * d->tri_list},
+        {(void **)&c->p.tri_lval, sizeof(double) * N * d->tri_list},
+        {(void **)&c->p.kg_lst, sizeof(KgonCut) * N * d->kg_list},
+        {(void **)&c->p.kg_lval, sizeof(double) * N * d->kg_list},
+        {(void **)&c->p.csr_ptr, sizeof(int) * N * (d->npairs + 1)},
+        {(void **)&c->p.csr_cut, sizeof(int) * N * d->max_contrib},
+        {(void **)&c->p.csr_fill, sizeof(int) * N * d->npairs},
+        {(void **)&c->p.fracsol, sizeof(double) * N * B},
+        {(void **)&c->p.xbest, sizeof(int) * N * B},
+        {(void **)&c->p.xheur, sizeof(int) * N * B},
+        {(void **)&c->p.perm, sizeof(int) * N * BQB_SA_CHAINS * d->ld},
+        {(void **)&c->p.sa_val, sizeof(double) * N * BQB_SA_CHAINS},
+        {(void **)&c->p.sa_perm, sizeof(int) * N * BQB_SA_CHAINS * 7},
+        {(void **)&c->p.sa_status, sizeof(int) * N * BQB_SA_CHAINS},
+    };
+    const int nitems = (int)(sizeof items / sizeof items[0]);
+    size_t total = 0;
+    for (int i = 0; i < nitems; ++i) total += align_up(items[i].bytes);
+    CK(hipMalloc(&c->arena, total), "hipMalloc arena");
+    c->arena_bytes = total;
+    CK(hipMemset(c->arena, 0, total), "hipMemset arena");
+    {
+        char *base = (char *)c->arena;
+        for (int i = 0; i < nitems; ++i) {
+            *items[i].ptr = base;
+            base += align_up(items[i].bytes);
+        }
+    }
+    CK(hipMemcpy((void *)c->p.SP_L, SP_L, sizeof(double) * (size_t)n0 * n0, hipMemcpyHostToDevice), "H2D SP_L");
+    CK(hipMemcpy(c->p.params, &c->params, sizeof(BqbParams), hipMemcpyHostToDevice), "H2D params");
+    CK(hipStreamCreate(&c->stream), "hipStreamCreate");
+    CK(hipEventCreate(&c->ev0), "hipEventCreate");
+    CK(hipEventCreate(&c->ev1), "hipEventCreate");
+    c->h_st = calloc(N, sizeof(NodeState));
+    if (!c->h_st) { bqb_set_error("calloc"); goto fail; }
+    return c;
+fail:
+    if (c->arena) hipFree(c->arena);
+    free(c->h_st);
+    free(c);
+    return NULL;
+}
+
+void bqb_ctx_destroy(bqb_ctx *c)
+{
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
+    hipFree(c->arena);
+    hipStreamDestroy(c->stream);
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    free(c->h_st);
+    free(c);
+}
+
+size_t bqb_ctx_bytes(const bqb_ctx *c) { return c->arena_bytes; }
+double bqb_ctx_last_ms(const bqb_ctx *c) { return c->last_ms; }
+int bqb_ctx_ld(const bqb_ctx *c) { return c->d.ld; }
+
+static int ipm_launch(bqb_ctx *c, const double *L, double *X)
+{
+    IpmBatchArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = c->p.nvec;
+    a.L = L;
+    a.X = X;
+    a.work = c->p.ipm_work;
+    a.ld = c->d.ld;
+    a.phi = c->p.phi;
+    a.iters = c->p.ipm_iters;
+    a.trials = c->p.ipm_trials;
+    a.status = c->p.ipm_status;
+    a.active = c->p.mask;
+    if (c->profile_ipm) hipEventRecord(c->ev0, c->stream);
+    if (bqb_check(bqb_launch_ipm_batch(&a, c->d.nodes, c->tile, c->stream), "ipm launch")) return -1;
+    if (c->profile_ipm) {
+        float ms = 0.f;
+        hipEventRecord(c->ev1, c->stream);
+        hipEventSynchronize(c->ev1);
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        c->ipm_ms += ms;
+    }
+    return 0;
+}
+
+/* Run one phase kernel on the current batch (also used by tests). */
+int bqb_ctx_run(bqb_ctx *c, int what, int arg)
+{
+    const BatchDims *d = &c->d;
+    const BatchPtrs *p = &c->p;
+    hipStream_t s = c->stream;
+    switch (what) {
+    case 100: return ipm_launch(c, p->L, p->X);
+    case 101: return ipm_launch(c, p->L0, p->Xt);
+    case 102: return bqb_check(bqb_launch_heur(d, p, c->tile, s), "heur launch");
+    case 103: return bqb_check(bqb_launch_trisep(d, p, s), "trisep launch");
+    case 104: return bqb_check(bqb_launch_sa(d, p, 5, s), "sa5 launch");
+    case 105: return bqb_check(bqb_launch_sa(d, p, 7, s), "sa7 launch");
+    case 106: return bqb_check(bqb_launch_csr_build(d, p, s), "csr launch");
+    case 107: return bqb_check(bqb_launch_opbt(d, p, arg, s), "opbt launch");
+    case 108: return bqb_check(bqb_launch_bundle_qp(d, p, s), "qp launch");
+    case 109: return bqb_check(bqb_launch_bundle_step(d, p, s), "step launch");
+    default:
+        if (what >= 0 && what <= PH_FINISH)
+            return bqb_check(bqb_launch_control(d, p, what, arg, s), "control launch");
+        bqb_set_error("unknown phase %d", what);
+        return -1;
+    }
+}
+
+static int count_active(bqb_ctx *c)
+{
+    if (bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * c->d.nodes,
+                                 hipMemcpyDeviceToHost, c->stream), "D2H state"))
+        return -1;
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    int a = 0;
+    for (int i = 0; i < c->d.nodes; ++i) a += c->h_st[i].active ? 1 : 0;
+    return a;
+}
+
+#define RUN(what, arg) do { if (bqb_ctx_run(c, (what), (arg))) return -1; } while (0)
+
+/* Upload a batch: fixings and the per-node inputs of the bound. */
+int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
+                 const double *diff, const int *is_root, const uint64_t *seeds)
+{
+    if (nodes < 1 || nodes > c->max_nodes) { bqb_set_error("batch of %d nodes (capacity %d)", nodes, c->max_nodes); return -1; }
+    c->d.nodes = nodes;
+    const size_t B = (size_t)(c->d.n0 - 1);
+    memset(c->h_st, 0, sizeof(NodeState) * nodes);
+    for (int i = 0; i < nodes; ++i) {
+        c->h_st[i].LB = LB ? LB[i] : 0.0;
+        c->h_st[i].diff = diff ? diff[i] : 0.0;
+        c->h_st[i].is_root = is_root ? is_root[i] : 0;
+        c->h_st[i].seed = seeds ? seeds[i] : (uint64_t)i;
+    }
+    if (bqb_check(hipMemcpyAsync((void *)c->p.xfixed, xfixed, sizeof(int) * nodes * B, hipMemcpyHostToDevice, c->stream), "H2D xfixed")) return -1;
+    if (bqb_check(hipMemcpyAsync((void *)c->p.solX, solX, sizeof(int) * nodes * B, hipMemcpyHostToDevice, c->stream), "H2D solX")) return -1;
+    if (bqb_check(hipMemcpyAsync(c->p.st, c->h_st, sizeof(NodeState) * nodes, hipMemcpyHostToDevice, c->stream), "H2D state")) return -1;
+    return bqb_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+/* The phase sequence of bounding.c:24-342 for the loaded batch. */
+int bqb_ctx_bound(bqb_ctx *c)
+{
+    const BqbParams *P = &c->params;
+    RUN(PH_SETUP, 0);
+    RUN(100, 0);                 /* basic IPM: L -> X */
+    RUN(PH_POST_BASIC, 0);
+    RUN(102, 0);                 /* heuristic */
+    RUN(PH_CHECK_BASIC, 0);
+    RUN(103, 0);                 /* first triangle separation */
+    RUN(PH_BUNDLE_INIT, 0);
+    RUN(106, 0);                 /* op_Bt gather lists */
+    int bdl = P->init_bundle_iter;
+    const int max_count = P->max_outer_iter + P->extra_iter;
+    for (int count = 1; count <= max_count; ++count) {
+        RUN(PH_OUTER_BEGIN, bdl);
+        for (int b = 0; b < bdl; ++b) {
+            RUN(108, 0);         /* QP -> gamma_test */
+            RUN(107, 1);         /* L0 = L - B'(gamma_test) */
+            RUN(101, 0);         /* IPM: L0 -> X_test */
+            RUN(109, 0);         /* serious / null step */
+        }
+        RUN(PH_OUTER_A, 0);
+        RUN(102, 0);             /* heuristic */
+        RUN(PH_OUTER_B, 0);
+        RUN(103, 0);             /* triangles */
+        RUN(PH_SEP_PENT, 0);
+        RUN(104, 0);             /* pentagonal */
+        RUN(PH_SEP_HEPT, 0);
+        RUN(105, 0);             /* heptagonal */
+        RUN(PH_OUTER_C, 0);
+        RUN(106, 0);             /* new gather lists */
+        RUN(107, 0);             /* L0 = L - B'(gamma) */
+        RUN(101, 0);             /* IPM: L0 -> X_test */
+        RUN(PH_REBUILD, 0);
+        bdl += count % 2;
+        if (bdl > P->max_bundle_iter) bdl = P->max_bundle_iter;
+        const int a = count_active(c);
+        if (a < 0) return -1;
+        if (a == 0) break;
+    }
+    RUN(PH_FINISH, 0);
+    return 0;
+}
+
+int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, double *LB_out,
+                    int *improved, double *diff_out, int *status, int *stats)
+{
+    const int nodes = c->d.nodes;
+    const size_t B = (size_t)(c->d.n0 - 1);
+    if (bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * nodes, hipMemcpyDeviceToHost, c->stream), "D2H state")) return -1;
+    if (fracsol && bqb_check(hipMemcpyAsync(fracsol, c->p.fracsol, sizeof(double) * nodes * B, hipMemcpyDeviceToHost, c->stream), "D2H fracsol")) return -1;
+    if (xbest && bqb_check(hipMemcpyAsync(xbest, c->p.xbest, sizeof(int) * nodes * B, hipMemcpyDeviceToHost, c->stream), "D2H xbest")) return -1;
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    for (int i = 0; i < nodes; ++i) {
+        const NodeState *s = &c->h_st[i];
+        if (bound) bound[i] = s->bound;
+        if (LB_out) LB_out[i] = s->LB;
+        if (improved) improved[i] = s->improved;
+        if (diff_out) diff_out[i] = s->diff;
+        if (status) status[i] = s->status;
+        if (stats) {
+            int *o = stats + 8 * i;
+            o[0] = s->prune; o[1] = s->giveup; o[2] = s->count; o[3] = s->ipm_calls;
+            o[4] = s->ipm_iters; o[5] = s->max_cuts; o[6] = s->max_bundle_seen; o[7] = s->n;
+        }
+    }
+    return 0;
+}
+
+int bqb_bound_batch(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
+                    const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
+                    double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
+                    int *status, int *stats)
+{
+    if (bqb_ctx_load(c, nodes, xfixed, solX, LB, diff, is_root, seeds)) return -1;
+    float ms = 0.f;
+    hipEventRecord(c->ev0, c->stream);
+    if (bqb_ctx_bound(c)) return -1;
+    hipEventRecord(c->ev1, c->stream);
+    if (bqb_check(hipEventSynchronize(c->ev1), "sync")) return -1;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    return bqb_ctx_results(c, bound, fracsol, xbest, LB_out, improved, diff_out, status, stats);
+}
+
+/* ---- raw access to one node's device buffers (tests, diagnostics) ---- */
+static int buffer_of(bqb_ctx *c, int id, int node, char **ptr, size_t *bytes)
+{
+    const BatchDims *d = &c->d;
+    const BatchPtrs *p = &c->p;
+    const size_t slab = (size_t)d->ld * d->ld, B = (size_t)(d->n0 - 1);
+    switch (id) {
+    case 0: *ptr = (char *)(p->st + node); *bytes = sizeof(NodeState); break;
+    case 1: *ptr = (char *)(p->L + node * slab); *bytes = sizeof(double) * slab; break;
+    case 2: *ptr = (char *)(p->L0 + node * slab); *bytes = sizeof(double) * slab; break;
+    case 3: *ptr = (char *)(p->X + node * slab); *bytes = sizeof(double) * slab; break;
+    case 4: *ptr = (char *)(p->Xt + node * slab); *bytes = sizeof(double) * slab; break;
+    case 5: *ptr = (char *)(p->tri + (size_t)node * d->max_tri); *bytes = sizeof(TriCut) * d->max_tri; break;
+    case 6: *ptr = (char *)(p->tri_y + (size_t)node * d->max_tri); *bytes = sizeof(double) * d->max_tri; break;
+    case 7: *ptr = (char *)(p->pent + (size_t)node * d->max_pent); *bytes = sizeof(KgonCut) * d->max_pent; break;
+    case 8: *ptr = (char *)(p->pent_y + (size_t)node * d->max_pent); *bytes = sizeof(double) * d->max_pent; break;
+    case 9: *ptr = (char *)(p->hept + (size_t)node * d->max_hept); *bytes = sizeof(KgonCut) * d->max_hept; break;
+    case 10: *ptr = (char *)(p->hept_y + (size_t)node * d->max_hept); *bytes = sizeof(double) * d->max_hept; break;
+    case 11: *ptr = (char *)(p->gamma + (size_t)node * d->max_m); *bytes = sizeof(double) * d->max_m; break;
+    case 12: *ptr = (char *)(p->gtest + (size_t)node * d->max_m); *bytes = sizeof(double) * d->max_m; break;
+    case 13: *ptr = (char *)(p->g + (size_t)node * d->max_m); *bytes = sizeof(double) * d->max_m; break;
+    case 14: *ptr = (char *)(p->G + (size_t)node * d->max_bundle * d->max_m); *bytes = sizeof(double) * d->max_bundle * d->max_m; break;
+    case 15: *ptr = (char *)(p->F + (size_t)node * d->max_bundle); *bytes = sizeof(double) * d->max_bundle; break;
+    case 16: *ptr = (char *)(p->lambda + (size_t)node * d->max_bundle); *bytes = sizeof(double) * d->max_bundle; break;
+    case 17: *ptr = (char *)(p->tri_lst + (size_t)node * d->tri_list); *bytes = sizeof(TriCut) * d->tri_list; break;
+    case 18: *ptr = (char *)(p->tri_lval + (size_t)node * d->tri_list); *bytes = sizeof(double) * d->tri_list; break;
+    case 19: *ptr = (char *)(p->kg_lst + (size_t)node * d->kg_list); *bytes = sizeof(KgonCut) * d->kg_list; break;
+    case 20: *ptr = (char *)(p->kg_lval + (size_t)node * d->kg_list); *bytes = sizeof(double) * d->kg_list; break;
+    case 21: *ptr = (char *)(p->fracsol + (size_t)node * B); *bytes = sizeof(double) * B; break;
+    case 22: *ptr = (char *)(p->xbest + (size_t)node * B); *bytes = sizeof(int) * B; break;
+    case 23: *ptr = (char *)(p->xheur + (size_t)node * B); *bytes = sizeof(int) * B; break;
+    case 24: *ptr = (char *)(p->mask + node); *bytes = sizeof(int); break;
+    case 25: *ptr = (char *)(p->phi + node); *bytes = sizeof(double); break;
+    case 26: *ptr = (char *)(p->Xb + (size_t)node * d->max_bundle * slab); *bytes = sizeof(double) * d->max_bundle * slab; break;
+    case 27: *ptr = (char *)(p->nvec + node); *bytes = sizeof(int); break;
+    case 28: *ptr = (char *)(p->dgamma + (size_t)node * d->max_m); *bytes = sizeof(double) * d->max_m; break;
+    case 29: *ptr = (char *)(p->sa_val + (size_t)node * BQB_SA_CHAINS); *bytes = sizeof(double) * BQB_SA_CHAINS; break;
+    default: bqb_set_error("unknown buffer %d", id); return -1;
+    }
+    return 0;
+}
+
+int bqb_ctx_download(bqb_ctx *c, int id, int node, void *host, size_t bytes)
+{
+    char *ptr; size_t cap;
+    if (node < 0 || node >= c->max_nodes || buffer_of(c, id, node, &ptr, &cap)) return -1;
+    if (bytes > cap) bytes = cap;
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    return bqb_check(hipMemcpy(host, ptr, bytes, hipMemcpyDeviceToHost), "D2H buffer");
+}
+
+int bqb_ctx_upload(bqb_ctx *c, int id, int node, const void *host, size_t bytes)
+{
+    char *ptr; size_t cap;
+    if (node < 0 || node >= c->max_nodes || buffer_of(c, id, node, &ptr, &cap)) return -1;
+    if (bytes > cap) { bqb_set_error("upload larger than buffer"); return -1; }
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    return bqb_check(hipMemcpy(ptr, host, bytes, hipMemcpyHostToDevice), "H2D buffer");
+}
+
+int bqb_ctx_sync(bqb_ctx *c) { return bqb_check(hipStreamSynchronize(c->stream), "sync"); }
+
+void bqb_ctx_profile_ipm(bqb_ctx *c, int on) { c->profile_ipm = on; c->ipm_ms = 0.0; }
+double bqb_ctx_ipm_ms(const bqb_ctx *c) { return c->ipm_ms; }

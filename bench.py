@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Batched node-bounding benchmark on one MI355X.
+
+Workload (batched bounding): Max-Cut on G(n=100, p=0.3) with weights uniform
+in {-1,+1}; 512 B&B subproblems per launch, each with 0-40 of the 99 branching
+variables fixed uniformly at random (subproblem sizes 60-100), bounded with the
+full SDPbound (FP64 IPM + bundle over triangle / pentagonal / heptagonal cuts,
+GW heuristic).  Every node receives the incumbent and the root gap of the
+instance's root bound, as a B&B worker does.
+
+One step = one bqb_bound_batch call over the 512 nodes (inputs uploaded inside
+the call, a few hundred KB; the timed region includes it).  value = node
+bounds per second.  The roofline object covers the dominant kernel, the batched
+IPM: algorithmic FP64 flops (6 n^3 per iteration + n^3/3 per line-search
+Cholesky test, counted on the device) over the summed duration of its launches
+(HIP event pairs on the launch stream, during the timed steps).  The
+cpu_baseline times the CPU oracle (the FP64 restatement of the same
+algorithm) on a bounded sample of the same nodes with one thread per core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N0 = 100                # graph vertices (the last one is fixed: 99 branching variables)
+EDGE_P = 0.3
+NFIX_MAX = 40
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ipm_batch_cfg2_traffic.json")
+
+
+def node_seeds(seed, count):
+    return (np.uint64(seed) * np.uint64(1000003) + np.arange(1, count + 1, dtype=np.uint64)).astype(np.uint64)
+
+
+def build_workload(api, I, seed, nodes):
+    adj = I.gnp_adj(N0, EDGE_P, seed, weights="pm1")
+    SP = api.maxcut_L(adj)
+    P = api.make_params()                      # the reference's shipped params file
+    if P.adjust_TriIneq:
+        P.TriIneq = 10 * N0                     # src/process_input.c:135-136
+    xf, sx = I.random_fixings(N0, NFIX_MAX, nodes, seed + 1)
+    return SP, P, xf, sx, node_seeds(seed, nodes)
+
+
+def cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample):
+    """The CPU oracle on the first `sample` nodes, one thread per core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    cores = min(16, len(os.sched_getaffinity(0)))
+    d = {f: getattr(P, f) for f, _ in type(P)._fields_}
+    op = O.default_params(**d)
+    t0 = time.perf_counter()
+    bounds, status = O.sdpbound_many(SP, op, xf[:sample], sx[:sample], LB, diff, seeds[:sample], cores)
+    dt = time.perf_counter() - t0
+    return {"value": sample / dt, "unit": "bounds/s", "cores": cores, "kind": "port",
+            "sample": f"first {sample} of the {len(xf)} nodes (same instance, incumbent and root gap), "
+                      f"{dt:.1f} s wall on {cores} threads"}, bounds, status
+
+
+def traffic_per_launch():
+    """HBM bytes per IPM launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload, or None."""
+    try:
+        t = json.load(open(TRAFFIC_FILE))
+        return float(t["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nodes", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--cpu-sample", type=int, default=192)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build bounds on one GPU; --gpus {args.gpus} is not supported "
+              "(the node queue is single-process; sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from parallel_biqbin_maxcut_amd import api, instances as I
+
+    SP, P, xf, sx, seeds = build_workload(api, I, args.seed, args.nodes)
+    ctx = api.BoundContext(SP, params=P, max_nodes=args.nodes)
+    # root bound of the instance: incumbent and root gap the workers receive
+    z = np.zeros((1, N0 - 1), np.int32)
+    root = ctx.bound(z, z, LB=0.0, is_root=1, seeds=np.uint64(args.seed))
+    if root["status"][0]:
+        raise SystemExit(f"root bound failed with status {root['status'][0]}")
+    LB, diff = float(root["LB"][0]), float(root["diff"][0])
+
+    for _ in range(args.warmup):
+        ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds)
+    ctx.profile_ipm(max_launches=2000 * max(1, args.steps))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    launches, ipm_ms, flops = ctx.ipm_stats()
+    ctx.profile_ipm(0)
+    if np.any(r["status"]):
+        raise SystemExit(f"node bound failures: {np.unique(r['status'], return_counts=True)}")
+
+    ms_per_step = dt / args.steps * 1e3
+    value = args.nodes * args.steps / dt
+    achieved = flops / (ipm_ms / 1e3) / 1e12 if ipm_ms > 0 else 0.0
+    stats = r["stats"]
+    line = {
+        "metric": "node bounds/sec (batched bounding, plain FP64 GPU form)",
+        "value": round(value, 2),
+        "unit": "bounds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched bounding: Max-Cut G(n=100, p=0.3), weights in {-1,+1}, 512 subproblems "
+                        "per launch with 0-40 variables fixed at random (sizes 60-100), FP64 IPM + bundle "
+                        "with triangle/pentagonal/heptagonal cuts",
+            "nodes_per_launch": args.nodes,
+            "seed": args.seed,
+            "incumbent": LB,
+            "root_gap": round(diff, 4),
+            "pruned_after_basic_bound": int(np.sum((stats[:, 0] == 1) & (stats[:, 2] == 0))),
+            "mean_outer_rounds": round(float(stats[:, 2].mean()), 3),
+            "ipm_calls": int(stats[:, 3].sum()),
+        },
+        "roofline": {
+            "kernel": "ipm_batch_kernel",
+            "bound": "mfma",
+            "achieved": round(achieved, 4),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+            "traffic": traffic_per_launch(),
+            "launches": launches,
+            "avg_launch_ms": round(ipm_ms / max(launches, 1), 5),
+            "gflop_per_launch": round(flops / max(launches, 1) / 1e9, 4),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        sample = min(args.cpu_sample, args.nodes)
+        cb, cbounds, cstatus = cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample)
+        line["cpu_baseline"] = cb
+        ok = cstatus == 0
+        rel = np.abs(r["bound"][:sample][ok] - cbounds[ok]) / np.abs(cbounds[ok])
+        line["cpu_baseline"]["max_rel_bound_diff_vs_gpu"] = float(rel.max()) if rel.size else None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -68,9 +68,87 @@ typedef struct BiqBinParameters {
 int bqb_init(int device);                 /* select the HIP device (default 0) */
 const char *bqb_last_error(void);
 
+/* Binary layout of the reference's Problem and BabNode
+ * (src/biqbin_cpp_api.h:27-63, NMAX = 1024). */
+#define BQB_NMAX 1024
+typedef struct BqbProblem {
+    double *L;          /* objective matrix, n x n column-major */
+    int n;
+    int NIneq;
+    int NPentIneq;
+    int NHeptaIneq;
+    int bundle;
+} BqbProblem;
+typedef struct BqbBabNode {
+    int xfixed[BQB_NMAX];       /* 1 where the variable is fixed */
+    int solX[BQB_NMAX];         /* BabSolution sol: fixed values */
+    double fracsol[BQB_NMAX];   /* written by the bound (branching) */
+    int level;
+    double upper_bound;
+} BqbBabNode;
+
 /* ---- drop-in: reference signatures ------------------------------------ */
+/* A host built from the reference's main.c, bab_functions.c, heap.c,
+ * process_input.c and allocate_free.c links this library in place of the
+ * files of the bounding path.  The incumbent and root gap are read and
+ * raised through that host's Bab_LBGet()/updateSolution() and its globals
+ * params/diff (weak references; a host without them gets library-internal
+ * ones, see bqb_dropin_incumbent).  The declarations are skipped when the
+ * reference's own biqbin.h (include guard BIQBIN_H) is already included, so
+ * its declarations, which have the same ABI, are the ones in force. */
+#ifndef BIQBIN_H
 /* replaces src/ipm_mc_pk.c:28 (declared src/biqbin.h:176) */
 void ipm_mc_pk(double *L, int n, double *X, double *phi, int print);
+/* replaces src/evaluate.c:11 (src/biqbin.h:155): createSubproblem + SDPbound */
+double Evaluate(BqbBabNode *node, BqbProblem *SP, BqbProblem *PP, int rank);
+/* replaces src/bounding.c:24 (src/biqbin.h:137) */
+double SDPbound(BqbBabNode *node, BqbProblem *SP, BqbProblem *PP, int rank);
+/* replaces src/evaluate.c:38 (src/biqbin.h:156) */
+void createSubproblem(BqbBabNode *node, BqbProblem *SP, BqbProblem *PP);
+/* replaces src/evaluate.c:123 (src/biqbin.h:157) */
+double getFixedValue(BqbBabNode *node, BqbProblem *SP);
+#endif
+/* The library-internal incumbent of the drop-in entry points (value; sol
+ * receives n0-1 entries if not NULL) and release of their device context. */
+double bqb_dropin_incumbent(int *sol);
+void bqb_dropin_release(void);
+
+/* ---- host layer: input, objective, node queue ---------------------------- */
+/* Defaults of the reference's PARAM_FIELDS (src/biqbin.h:49-78). */
+void bqb_params_default(BiqBinParameters *p);
+/* replaces readParameters (src/process_input.c:153-180): defaults, then
+ * "name = value" lines.  0 on success. */
+int bqb_read_params(const char *path, BiqBinParameters *p);
+/* replaces readData (src/process_input.c:206-250): rudy format, 1-based
+ * vertices, integer weights.  Returns a malloc'd nv x nv adjacency matrix
+ * (free with free()) or NULL. */
+double *bqb_read_rudy(const char *path, int *nv);
+/* replaces process_adj_matrix (src/process_input.c:259-344): SP->L of an
+ * nv-vertex graph (nv x nv, column-major); the last vertex is the one fixed
+ * to 0, so the B&B has nv-1 variables. */
+int bqb_build_L(int nv, const double *adj, double *L);
+/* replaces evaluateSolution (src/bab_functions.c:96-107) for a given SP->L. */
+double bqb_evaluate_solution(int n0, const double *SP_L, const int *sol);
+
+typedef struct bqb_solve_stats {
+    double best;          /* best cut value found (proven optimal unless stopped) */
+    double root_bound;
+    long long nodes;      /* evaluated nodes, root included */
+    long long batches;    /* bound launches */
+    int status;           /* first non-zero node status */
+    int stopped;          /* params.time_limit reached */
+    int max_queue;
+    double seconds;       /* wall time */
+    double bound_ms;      /* time inside the batched bound (device events) */
+} bqb_solve_stats;
+
+/* Single-process branch and bound on one GPU; replaces the MPI
+ * master/worker loop (src/main.c, src/bab_functions.c:39-69, 224-297) with
+ * one node queue: up to `batch` best nodes are bounded per launch against
+ * the incumbent at that moment.  sol receives n0-1 entries (the cut side of
+ * every vertex but the last, which is on side 0).  Returns 0 on success. */
+int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
+              int *sol, bqb_solve_stats *stats);
 
 /* ---- batch: interior-point solve of the basic SDP ---------------------- */
 /* Solves max <L_b, X_b> s.t. diag(X_b) = e, X_b psd for `nodes` independent
@@ -130,8 +208,12 @@ int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, doub
 int bqb_ctx_download(bqb_ctx *c, int id, int node, void *host, size_t bytes);
 int bqb_ctx_upload(bqb_ctx *c, int id, int node, const void *host, size_t bytes);
 int bqb_ctx_sync(bqb_ctx *c);
-void bqb_ctx_profile_ipm(bqb_ctx *c, int on);
-double bqb_ctx_ipm_ms(const bqb_ctx *c);
+/* IPM launch profile (no counterpart in the reference; for benchmarks):
+ * start with max_launches > 0 (event pair per IPM launch, device flop
+ * counter reset), stop with 0; read launches, summed ms and algorithmic
+ * flops (6 n^3 per iteration + n^3/3 per line-search Cholesky test). */
+int bqb_ctx_profile_ipm(bqb_ctx *c, int max_launches);
+int bqb_ctx_ipm_stats(bqb_ctx *c, int *launches, double *ms_total, double *flops);
 
 /* Diagnostic: one matrix, stop after max_iters iterations and return the
  * kernel's internal inv(Z) (Zi), inv(Zi o X) (Mi) and search direction dX of

@@ -212,6 +212,15 @@ void orc_bab_solve(const double *SP_L, int n0, const orc_params *p, uint64_t see
 
 void orc_params_default(orc_params *p);
 
+/* Attach a step trace to a workspace (rows of 8 doubles; cap rows) and read
+ * how many rows the last orc_sdpbound wrote.  Row tags:
+ *   1 bundle step   (count, f_test, f, t, bundle, m, serious)
+ *   2 QP            (count, t, lambda_0, sum dgamma^2, bundle, m, 0)
+ *   3 outer round   (count, NIneq, NPentIneq, NHeptaIneq, viol3, LB, bound)
+ *   4 heuristic     (count, LB, value, 0, 0, 0, 0)                       */
+void orc_ws_set_trace(orc_ws *w, double *buf, int cap);
+int orc_ws_trace_n(const orc_ws *w);
+
 /* Test helper: the heuristic of one node on a given X (see orc_heur.c). */
 double orc_heuristic_on(const double *SP_L, int n0, const int *xfixed, const int *solX,
                         const double *PP_L, const double *X, int n, uint64_t key, int *x_out,
@@ -220,7 +229,8 @@ double orc_heuristic_on(const double *SP_L, int n0, const int *xfixed, const int
 /* Bound many independent nodes with up to nthreads threads (cpu baseline). */
 void orc_sdpbound_many(const double *SP_L, int n0, const orc_params *p, int count,
                        const int *xfixed, const int *solX, const double *LB,
-                       const uint64_t *seeds, int nthreads, double *bounds, int *status);
+                       const double *diff, const uint64_t *seeds, int nthreads, double *bounds,
+                       int *status);
 
 #ifdef __cplusplus
 }

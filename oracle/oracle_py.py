@@ -108,7 +108,7 @@ def lib():
         L.orc_sdpbound.argtypes = [C.c_void_p, dptr, C.c_int, C.POINTER(NodeIn), C.POINTER(NodeOut), dptr, iptr]
         L.orc_bab_solve.argtypes = [dptr, C.c_int, C.POINTER(Params), C.c_uint64, C.c_int, iptr, C.POINTER(BabOut)]
         L.orc_params_default.argtypes = [C.POINTER(Params)]
-        L.orc_sdpbound_many.argtypes = [dptr, C.c_int, C.POINTER(Params), C.c_int, iptr, iptr, dptr,
+        L.orc_sdpbound_many.argtypes = [dptr, C.c_int, C.POINTER(Params), C.c_int, iptr, iptr, dptr, dptr,
                                         C.POINTER(C.c_uint64), C.c_int, dptr, iptr]
         L.orc_key.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_key.restype = C.c_uint64
@@ -116,6 +116,9 @@ def lib():
         L.orc_draw.restype = C.c_uint64
         L.orc_heuristic_on.argtypes = [dptr, C.c_int, iptr, iptr, dptr, dptr, C.c_int, C.c_uint64, iptr, iptr]
         L.orc_heuristic_on.restype = C.c_double
+        L.orc_ws_set_trace.argtypes = [C.c_void_p, dptr, C.c_int]
+        L.orc_ws_trace_n.argtypes = [C.c_void_p]
+        L.orc_ws_trace_n.restype = C.c_int
         L.orc_chol_upper.argtypes = [dptr, C.c_int]
         L.orc_chol_upper.restype = C.c_int
         _LIB = L
@@ -267,6 +270,37 @@ def sdpbound(SP_L, xfixed, solX, LB, diff, is_root, seed, params, ws=None):
     if own:
         lib().orc_ws_destroy(ws)
     return out, frac, xb
+
+
+def sdpbound_trace(SP_L, xfixed, solX, LB, diff, is_root, seed, params, cap=4096):
+    """sdpbound with the step trace: returns (out, frac, xbest, rows[r, 8])."""
+    SP_L = f64(SP_L)
+    ws = lib().orc_ws_create(SP_L.shape[0], 400, 50000, 50000, 50000, C.byref(params))
+    buf = np.zeros(8 * cap)
+    lib().orc_ws_set_trace(ws, _d(buf), cap)
+    try:
+        out, frac, xb = sdpbound(SP_L, xfixed, solX, LB, diff, is_root, seed, params, ws=ws)
+        rows = buf[: 8 * lib().orc_ws_trace_n(ws)].reshape(-1, 8).copy()
+    finally:
+        lib().orc_ws_destroy(ws)
+    return out, frac, xb, rows
+
+
+def sdpbound_many(SP_L, params, xfixed, solX, LB, diff, seeds, nthreads):
+    """Bound many nodes with nthreads CPU threads; returns (bounds, status)."""
+    SP_L = f64(SP_L)
+    n0 = SP_L.shape[0]
+    xf = np.ascontiguousarray(xfixed, dtype=np.int32)
+    sx = np.ascontiguousarray(solX, dtype=np.int32)
+    count = xf.shape[0]
+    lb = np.ascontiguousarray(np.broadcast_to(LB, (count,)), dtype=np.float64)
+    df = np.ascontiguousarray(np.broadcast_to(diff, (count,)), dtype=np.float64)
+    sd = np.ascontiguousarray(np.broadcast_to(seeds, (count,)), dtype=np.uint64)
+    bounds = np.zeros(count)
+    status = np.zeros(count, dtype=np.int32)
+    lib().orc_sdpbound_many(_d(SP_L), n0, C.byref(params), count, _i(xf), _i(sx), _d(lb), _d(df),
+                            sd.ctypes.data_as(C.POINTER(C.c_uint64)), nthreads, _d(bounds), _i(status))
+    return bounds, status
 
 
 def bab(SP_L, params, seed=2020, batch=1):

@@ -65,6 +65,14 @@ orc_ws *orc_ws_create(int n0, int max_bundle, int max_tri, int max_pent, int max
     return w;
 }
 
+void orc_ws_set_trace(orc_ws *w, double *buf, int cap)
+{
+    w->trace = buf;
+    w->trace_cap = buf ? cap : 0;
+    w->trace_n = 0;
+}
+int orc_ws_trace_n(const orc_ws *w) { return w->trace_n; }
+
 void orc_ws_destroy(orc_ws *w)
 {
     if (!w) return;
@@ -281,6 +289,8 @@ void orc_sdpbound(orc_ws *w, const double *SP_L, int n0, const orc_node_in *in,
     w->improved = 0;
     w->status = 0;
     w->ipm_calls = w->ipm_iters = w->max_cuts = w->max_bundle_seen = 0;
+    w->trace_n = 0;
+    w->count = 0;
     double diff = in->diff;
     int prune = 0, giveup = 0, done = 0, count = 0;
     int bdl_iter = p->init_bundle_iter;
@@ -302,8 +312,11 @@ void orc_sdpbound(orc_ws *w, const double *SP_L, int n0, const orc_node_in *in,
     store_fracsol(w, in->xfixed, in->solX, fracsol);
 
     for (int i = 0; i < B; ++i) x[i] = in->xfixed[i] ? in->solX[i] : 0;
-    orc_run_heuristic(w, in->xfixed, in->solX, x, KEY_HEUR(in->seed, 0));
-    update_solution(w, x);
+    {
+        double hv = orc_run_heuristic(w, in->xfixed, in->solX, x, KEY_HEUR(in->seed, 0));
+        update_solution(w, x);
+        orc_trace(w, 4, 0, w->LB, hv, 0, 0, 0, 0);
+    }
     bound = w->f + fixedvalue;
     if (w->status) goto END;
     if (bound < w->LB + 1.0) { prune = 1; goto END; }
@@ -322,6 +335,7 @@ void orc_sdpbound(orc_ws *w, const double *SP_L, int n0, const orc_node_in *in,
 
     while (!done) {
         ++count;
+        w->count = count;
         oldf = w->f;
         orc_bundle_method(w, &t, bdl_iter);
         if (w->status) { bound = w->f + fixedvalue; goto END; }
@@ -329,8 +343,9 @@ void orc_sdpbound(orc_ws *w, const double *SP_L, int n0, const orc_node_in *in,
         prune = (bound < w->LB + 1.0) ? 1 : 0;
         if (!prune) {
             for (int i = 0; i < B; ++i) x[i] = in->xfixed[i] ? in->solX[i] : 0;
-            orc_run_heuristic(w, in->xfixed, in->solX, x, KEY_HEUR(in->seed, count));
+            double hv = orc_run_heuristic(w, in->xfixed, in->solX, x, KEY_HEUR(in->seed, count));
             update_solution(w, x);
+            orc_trace(w, 4, count, w->LB, hv, 0, 0, 0, 0);
             if (w->status) goto END;
             prune = (bound < w->LB + 1.0) ? 1 : 0;
         }
@@ -352,6 +367,7 @@ void orc_sdpbound(orc_ws *w, const double *SP_L, int n0, const orc_node_in *in,
             if (w->status) goto END;
         }
         done = prune || giveup;
+        orc_trace(w, 3, count, w->NIneq, w->NPentIneq, w->NHeptaIneq, viol3, w->LB, bound);
         store_fracsol(w, in->xfixed, in->solX, fracsol);
 
         if (!done) {

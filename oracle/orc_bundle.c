@@ -275,6 +275,11 @@ int orc_bundle_method(orc_ws *w, double *t, int bdl_iter)
         }
         int st = orc_lambda_eta(m, k, zeta, w->G, m, w->gamma, w->dgamma, w->lambda, w->eta, t);
         if (st) { w->status = st; break; }
+        {
+            double nd = 0.0;
+            for (int i = 0; i < m; ++i) nd += w->dgamma[i] * w->dgamma[i];
+            orc_trace(w, 2, w->count, *t, w->lambda[0], nd, k, m, 0);
+        }
         for (int i = 0; i < m; ++i) w->gamma_test[i] = w->gamma[i] + w->dgamma[i];
         double f_test = orc_ws_fct_eval(w, w->gamma_test, w->X_test, w->g);
         if (w->status) break;
@@ -288,7 +293,8 @@ int orc_bundle_method(orc_ws *w, double *t, int bdl_iter)
         double lmax = -ORC_BIG_NUMBER;
         for (int i = 0; i < k; ++i) lmax = (w->lambda[i] > lmax) ? w->lambda[i] : lmax;
 
-        if (w->f - f_test > 0.05 * del) {       /* serious step */
+        const int serious = (w->f - f_test > 0.05 * del) ? 1 : 0;
+        if (serious) {                          /* serious step */
             memcpy(w->gamma, w->gamma_test, sizeof(double) * m);
             w->f = f_test;
             for (int i = 0; i < nn; ++i) {      /* X = X_bundle * lambda */
@@ -332,6 +338,7 @@ int orc_bundle_method(orc_ws *w, double *t, int bdl_iter)
             ++w->bundle;
         }
         if (w->bundle > w->max_bundle_seen) w->max_bundle_seen = w->bundle;
+        orc_trace(w, 1, w->count, f_test, w->f, *t, w->bundle, m, serious);
     }
     free(zeta);
     return w->status;

@@ -32,7 +32,21 @@ struct orc_ws {
     int improved;
     /* stats */
     int ipm_calls, ipm_iters, status, max_cuts, max_bundle_seen;
+    /* optional step trace (rows of ORC_TRACE_W doubles), for tests */
+    int count;                 /* current outer iteration (trace only) */
+    double *trace;
+    int trace_cap, trace_n;
 };
+
+#define ORC_TRACE_W 8
+/* Append one trace row (ignored when no trace buffer is attached). */
+static inline void orc_trace(orc_ws *w, double tag, double a, double b, double c, double d,
+                             double e, double f, double g)
+{
+    if (!w->trace || w->trace_n >= w->trace_cap) return;
+    double *r = w->trace + (size_t)ORC_TRACE_W * w->trace_n++;
+    r[0] = tag; r[1] = a; r[2] = b; r[3] = c; r[4] = d; r[5] = e; r[6] = f; r[7] = g;
+}
 
 int orc_ws_ensure_G(orc_ws *w, size_t need);
 /* bundle.c:9-239 on the workspace */

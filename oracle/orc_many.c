@@ -11,7 +11,8 @@
 
 void orc_sdpbound_many(const double *SP_L, int n0, const orc_params *p, int count,
                        const int *xfixed, const int *solX, const double *LB,
-                       const uint64_t *seeds, int nthreads, double *bounds, int *status)
+                       const double *diff, const uint64_t *seeds, int nthreads, double *bounds,
+                       int *status)
 {
     int B = n0 - 1;
     if (nthreads < 1) nthreads = 1;
@@ -22,7 +23,7 @@ void orc_sdpbound_many(const double *SP_L, int n0, const orc_params *p, int coun
         int *xb = malloc(sizeof(int) * B);
 #pragma omp for schedule(dynamic, 1)
         for (int i = 0; i < count; ++i) {
-            orc_node_in in = {xfixed + (size_t)i * B, solX + (size_t)i * B, LB[i], 0.0, 0, seeds[i]};
+            orc_node_in in = {xfixed + (size_t)i * B, solX + (size_t)i * B, LB[i], diff ? diff[i] : 0.0, 0, seeds[i]};
             orc_node_out o;
             orc_sdpbound(w, SP_L, n0, &in, &o, frac, xb);
             bounds[i] = o.bound;

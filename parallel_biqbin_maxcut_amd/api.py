@@ -1,6 +1,7 @@
 """Python mirror of the bounding entry points (calls the C ABI of
 libbiqbin_gpu.so; see include/biqbin_gpu.h)."""
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -8,6 +9,7 @@ from . import _native as N
 
 
 def declare(L):
+    _declare_host(L)
     L.bqb_ipm_debug.argtypes = [C.c_int, N.dptr, C.c_int, N.dptr, N.dptr, N.dptr, N.dptr, N.dptr,
                                 N.dptr, N.iptr, N.dptr, N.dptr]
     L.bqb_ipm_debug.restype = C.c_int
@@ -140,6 +142,105 @@ PH = dict(SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OU
           HEUR=102, TRISEP=103, PENT=104, HEPT=105, CSR=106, OPBT=107, QP=108, STEP=109)
 
 
+class SolveStats(C.Structure):
+    _fields_ = [("best", C.c_double), ("root_bound", C.c_double), ("nodes", C.c_longlong),
+                ("batches", C.c_longlong), ("status", C.c_int), ("stopped", C.c_int),
+                ("max_queue", C.c_int), ("seconds", C.c_double), ("bound_ms", C.c_double)]
+
+
+NMAX = 1024
+
+
+class Problem(C.Structure):
+    """Layout of the reference's Problem (src/biqbin_cpp_api.h:27-35)."""
+    _fields_ = [("L", C.POINTER(C.c_double)), ("n", C.c_int), ("NIneq", C.c_int),
+                ("NPentIneq", C.c_int), ("NHeptaIneq", C.c_int), ("bundle", C.c_int)]
+
+
+class BabNode(C.Structure):
+    """Layout of the reference's BabNode (src/biqbin_cpp_api.h:54-63)."""
+    _fields_ = [("xfixed", C.c_int * NMAX), ("solX", C.c_int * NMAX), ("fracsol", C.c_double * NMAX),
+                ("level", C.c_int), ("upper_bound", C.c_double)]
+
+
+def _declare_host(L):
+    L.bqb_params_default.argtypes = [C.POINTER(BiqBinParameters)]
+    L.bqb_params_default.restype = None
+    L.bqb_read_params.argtypes = [C.c_char_p, C.POINTER(BiqBinParameters)]
+    L.bqb_read_rudy.argtypes = [C.c_char_p, N.iptr]
+    L.bqb_read_rudy.restype = C.c_void_p
+    L.bqb_build_L.argtypes = [C.c_int, N.dptr, N.dptr]
+    L.bqb_evaluate_solution.argtypes = [C.c_int, N.dptr, N.iptr]
+    L.bqb_evaluate_solution.restype = C.c_double
+    L.bqb_solve.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, N.iptr,
+                            C.POINTER(SolveStats)]
+    for f in ("Evaluate", "SDPbound"):
+        getattr(L, f).argtypes = [C.POINTER(BabNode), C.POINTER(Problem), C.POINTER(Problem), C.c_int]
+        getattr(L, f).restype = C.c_double
+    L.createSubproblem.argtypes = [C.POINTER(BabNode), C.POINTER(Problem), C.POINTER(Problem)]
+    L.createSubproblem.restype = None
+    L.getFixedValue.argtypes = [C.POINTER(BabNode), C.POINTER(Problem)]
+    L.getFixedValue.restype = C.c_double
+    L.bqb_dropin_incumbent.argtypes = [N.iptr]
+    L.bqb_dropin_incumbent.restype = C.c_double
+    L.bqb_dropin_release.argtypes = []
+    L.bqb_dropin_release.restype = None
+
+
+def default_params():
+    """The reference's PARAM_FIELDS defaults (src/biqbin.h:49-78)."""
+    p = BiqBinParameters()
+    N.lib().bqb_params_default(C.byref(p))
+    return p
+
+
+def read_params(path):
+    """Parameter file as the reference reads it (src/process_input.c:153-180)."""
+    p = BiqBinParameters()
+    N.check(N.lib().bqb_read_params(os.fsencode(path), C.byref(p)))
+    return p
+
+
+def read_rudy(path):
+    """Adjacency matrix of a rudy graph file (src/process_input.c:206-250)."""
+    nv = C.c_int()
+    ptr = N.lib().bqb_read_rudy(os.fsencode(path), C.byref(nv))
+    if not ptr:
+        raise ValueError(N.lib().bqb_last_error().decode())
+    try:
+        a = np.ctypeslib.as_array(C.cast(ptr, N.dptr), shape=(nv.value * nv.value,)).copy()
+    finally:
+        C.CDLL(None).free(C.c_void_p(ptr))
+    return a.reshape((nv.value, nv.value), order="F")
+
+
+def maxcut_L(adj):
+    """SP->L of a graph (src/process_input.c:259-344), column-major."""
+    adj = np.asfortranarray(adj, dtype=np.float64)
+    nv = adj.shape[0]
+    L = np.zeros((nv, nv), order="F")
+    N.check(N.lib().bqb_build_L(nv, _d(adj), _d(L)))
+    return L
+
+
+def evaluate_solution(SP_L, sol):
+    SP_L = np.asfortranarray(SP_L, dtype=np.float64)
+    s = np.ascontiguousarray(sol, dtype=np.int32)
+    return N.lib().bqb_evaluate_solution(SP_L.shape[0], _d(SP_L), _i(s))
+
+
+def solve(SP_L, params=None, seed=2020, batch=64):
+    """Branch and bound on the GPU (bqb_solve).  Returns (stats, solution)
+    with the solution over the first n0-1 vertices (the last is on side 0)."""
+    SP_L = np.asfortranarray(SP_L, dtype=np.float64)
+    n0 = SP_L.shape[0]
+    p = params if params is not None else make_params()
+    sol = np.zeros(n0 - 1, dtype=np.int32)
+    st = SolveStats()
+    N.check(N.lib().bqb_solve(n0, _d(SP_L), C.byref(p), seed, batch, _i(sol), C.byref(st)))
+    return st, sol
+
+
 def _declare_ctx(L):
     vp = C.c_void_p
     L.bqb_ctx_create.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -162,9 +263,7 @@ def _declare_ctx(L):
     L.bqb_ctx_upload.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
     L.bqb_ctx_sync.argtypes = [vp]
     L.bqb_ctx_profile_ipm.argtypes = [vp, C.c_int]
-    L.bqb_ctx_profile_ipm.restype = None
-    L.bqb_ctx_ipm_ms.argtypes = [vp]
-    L.bqb_ctx_ipm_ms.restype = C.c_double
+    L.bqb_ctx_ipm_stats.argtypes = [vp, N.iptr, N.dptr, N.dptr]
 
 
 class BoundContext:
@@ -244,6 +343,22 @@ class BoundContext:
 
     def set_state(self, node, s):
         N.check(N.lib().bqb_ctx_upload(self.h, 0, node, C.addressof(s), C.sizeof(s)))
+
+    def sync(self):
+        N.check(N.lib().bqb_ctx_sync(self.h))
+
+    def profile_ipm(self, max_launches):
+        """Time every IPM launch (event pairs) and count its algorithmic
+        flops on the device, for up to max_launches launches; 0 stops."""
+        N.check(N.lib().bqb_ctx_profile_ipm(self.h, max_launches))
+
+    def ipm_stats(self):
+        """(launches, summed ms, algorithmic flops) since profiling started."""
+        k = C.c_int()
+        ms = C.c_double()
+        fl = C.c_double()
+        N.check(N.lib().bqb_ctx_ipm_stats(self.h, C.byref(k), C.byref(ms), C.byref(fl)))
+        return k.value, ms.value, fl.value
 
     def matrix(self, buf, node, n):
         a = self.get(buf, node, np.float64, self.ld * self.ld).reshape((self.ld, self.ld), order="F")

@@ -13,6 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbiqbin_gpu.so")
+CLI_SRC = os.path.join(HERE, "cli", "biqbin_gpu_main.c")
+CLI = os.path.join(HERE, "biqbin_gpu")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIPCC_FLAGS = [
@@ -60,8 +62,12 @@ def build(jobs=8, verbose=False):
     tmp = LIB + ".tmp"
     _run([os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # command-line solver, linked against the library next to it
+    _run(["gcc"] + CC_FLAGS + [CLI_SRC, "-o", CLI + ".tmp", "-L" + HERE, "-lbiqbin_gpu",
+                                "-Wl,-rpath,$ORIGIN"])
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, CLI)
     return LIB
 
 

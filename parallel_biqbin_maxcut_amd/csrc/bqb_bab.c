@@ -1,0 +1,382 @@
+/*
+ * bqb_bab.c -- host side of the solver (C): instance and parameter input,
+ * the objective matrix, and the branch-and-bound node queue that feeds the
+ * batched device bound.
+ *
+ * Reference behaviour followed:
+ *   src/process_input.c:153-180  readParameters ("name = value" lines over
+ *                                the PARAM_FIELDS defaults)
+ *   src/process_input.c:206-250  readData (rudy: "n m" then m lines "i j w",
+ *                                1-based, integer weights)
+ *   src/process_input.c:259-344  process_adj_matrix (SP->L)
+ *   src/process_input.c:135-136  adjust_TriIneq
+ *   src/bab_functions.c:25-69    initial incumbent 0, root node
+ *   src/bab_functions.c:224-297  evaluate, branch on the chosen variable
+ *   src/bab_functions.c:387-421  getBranchingVariable
+ *   src/heap.c                   max-heap on upper_bound (ties: -1)
+ *
+ * The MPI master/worker loop becomes one queue on one GPU: the best `batch`
+ * nodes are popped together (nodes whose bound no longer beats the
+ * incumbent are dropped), bounded in one bqb_bound_batch call against the
+ * incumbent at that moment, the incumbent is raised from their heuristic
+ * solutions in node order, and each survivor is branched.  Children carry
+ * the parent's bound until they are evaluated.  With batch = 1 this is the
+ * sequential B&B of the reference with one worker.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "bqb_batch.h"
+#include "bqb_rng.h"
+#include "bqb_runtime.h"
+
+/* ------------------------------------------------------------ parameters */
+void bqb_params_default(BiqBinParameters *p)
+{
+    p->init_bundle_iter = 3;
+    p->max_bundle_iter = 15;
+    p->triag_iter = 5;
+    p->pent_iter = 5;
+    p->hept_iter = 5;
+    p->max_outer_iter = 20;
+    p->extra_iter = 10;
+    p->violated_TriIneq = 1e-3;
+    p->TriIneq = 5000;
+    p->adjust_TriIneq = 1;
+    p->PentIneq = 5000;
+    p->HeptaIneq = 5000;
+    p->Pent_Trials = 60;
+    p->Hepta_Trials = 50;
+    p->include_Pent = 1;
+    p->include_Hepta = 1;
+    p->root = 0;
+    p->use_diff = 1;
+    p->time_limit = 0;
+    p->branchingStrategy = 1;
+}
+
+int bqb_read_params(const char *path, BiqBinParameters *p)
+{
+    bqb_params_default(p);
+    FILE *f = fopen(path, "r");
+    if (!f) { bqb_set_error("parameter file %s not found", path); return -1; }
+    char line[256], name[64];
+    while (fgets(line, sizeof line, f)) {
+        if (sscanf(line, " %63[^= \t]", name) != 1) continue;
+        const char *eq = strchr(line, '=');
+        if (!eq) continue;
+        const char *v = eq + 1;
+#define INT_FIELD(F) if (!strcmp(name, #F)) { p->F = atoi(v); continue; }
+        INT_FIELD(init_bundle_iter) INT_FIELD(max_bundle_iter) INT_FIELD(triag_iter)
+        INT_FIELD(pent_iter) INT_FIELD(hept_iter) INT_FIELD(max_outer_iter) INT_FIELD(extra_iter)
+        INT_FIELD(TriIneq) INT_FIELD(adjust_TriIneq) INT_FIELD(PentIneq) INT_FIELD(HeptaIneq)
+        INT_FIELD(Pent_Trials) INT_FIELD(Hepta_Trials) INT_FIELD(include_Pent)
+        INT_FIELD(include_Hepta) INT_FIELD(root) INT_FIELD(use_diff) INT_FIELD(time_limit)
+        INT_FIELD(branchingStrategy)
+#undef INT_FIELD
+        if (!strcmp(name, "violated_TriIneq")) { p->violated_TriIneq = atof(v); continue; }
+    }
+    fclose(f);
+    return 0;
+}
+
+/* ------------------------------------------------------------- instances */
+double *bqb_read_rudy(const char *path, int *nv_out)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) { bqb_set_error("cannot open %s", path); return NULL; }
+    int nv = 0, ne = 0;
+    if (fscanf(f, "%d %d", &nv, &ne) != 2 || nv <= 0 || ne < 0) {
+        fclose(f);
+        bqb_set_error("%s: bad header (expected 'vertices edges')", path);
+        return NULL;
+    }
+    double *adj = calloc((size_t)nv * nv, sizeof(double));
+    if (!adj) { fclose(f); bqb_set_error("calloc"); return NULL; }
+    for (int e = 0; e < ne; ++e) {
+        int i, j;
+        double w;
+        if (fscanf(f, "%d %d %lf", &i, &j, &w) != 3) {
+            bqb_set_error("%s: problem reading edge %d", path, e + 1);
+            goto bad;
+        }
+        if (i < 1 || i > nv || j < 1 || j > nv) {
+            bqb_set_error("%s: edge %d has a vertex out of range", path, e + 1);
+            goto bad;
+        }
+        if ((double)(int)w != w) {
+            bqb_set_error("%s: edge %d: all edge weights need to be integers", path, e + 1);
+            goto bad;
+        }
+        adj[(size_t)nv * (j - 1) + (i - 1)] = w;
+        adj[(size_t)nv * (i - 1) + (j - 1)] = w;
+    }
+    fclose(f);
+    *nv_out = nv;
+    return adj;
+bad:
+    fclose(f);
+    free(adj);
+    return NULL;
+}
+
+/* SP->L = [Lap, Lap e; (Lap e)', e' Lap e] over the first nv-1 vertices,
+ * where Lap = Diag(Adj e) - Adj uses the full degree (the last vertex is the
+ * one fixed to 0).  Column-major nv x nv. */
+int bqb_build_L(int nv, const double *adj, double *L)
+{
+    if (nv < 2) { bqb_set_error("need at least 2 vertices"); return -1; }
+    const int m = nv - 1;
+    double total = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double deg = 0.0;
+        for (int j = 0; j < nv; ++j) deg += adj[j + (size_t)i * nv];
+        double row = 0.0;
+        for (int j = 0; j < m; ++j) {
+            const double v = (i == j ? deg : 0.0) - adj[j + (size_t)i * nv];
+            L[j + (size_t)i * nv] = v;
+            row += v;
+        }
+        L[m + (size_t)i * nv] = row;     /* column i, last row */
+        total += row;
+    }
+    for (int j = 0; j < m; ++j) L[j + (size_t)m * nv] = L[m + (size_t)j * nv];
+    L[m + (size_t)m * nv] = total;
+    return 0;
+}
+
+/* x' L x over the leading (n0-1) x (n0-1) block (bab_functions.c:96-107). */
+double bqb_evaluate_solution(int n0, const double *SP_L, const int *sol)
+{
+    double v = 0.0;
+    for (int i = 0; i < n0 - 1; ++i)
+        for (int j = 0; j < n0 - 1; ++j) v += SP_L[j + (size_t)i * n0] * sol[i] * sol[j];
+    return v;
+}
+
+/* ------------------------------------------------------------ node queue */
+typedef struct qnode {
+    double upper_bound;
+    int level;
+    uint64_t id;
+    int *xfixed, *solX;
+    double *fracsol;
+} qnode;
+
+typedef struct qheap { qnode **d; long used, cap; } qheap;
+
+static int node_cmp(const qnode *a, const qnode *b) { return a->upper_bound > b->upper_bound ? 1 : -1; }
+
+static int heap_push(qheap *h, qnode *x)
+{
+    if (h->used == h->cap) {
+        const long cap = h->cap ? 2 * h->cap : 4096;
+        qnode **d = realloc(h->d, sizeof(qnode *) * cap);
+        if (!d) { bqb_set_error("node queue: out of memory"); return -1; }
+        h->d = d;
+        h->cap = cap;
+    }
+    long c = h->used++;
+    h->d[c] = x;
+    while (c > 0) {
+        const long par = (c - 1) / 2;
+        if (node_cmp(h->d[par], h->d[c]) >= 0) break;
+        qnode *t = h->d[par]; h->d[par] = h->d[c]; h->d[c] = t;
+        c = par;
+    }
+    return 0;
+}
+
+static qnode *heap_pop(qheap *h)
+{
+    qnode *top = h->d[0];
+    h->d[0] = h->d[--h->used];
+    long c = 0;
+    for (;;) {
+        long k = 2 * c + 1;
+        if (k >= h->used) break;
+        if (k + 1 < h->used && node_cmp(h->d[k + 1], h->d[k]) > 0) ++k;
+        if (node_cmp(h->d[c], h->d[k]) >= 0) break;
+        qnode *t = h->d[k]; h->d[k] = h->d[c]; h->d[c] = t;
+        c = k;
+    }
+    return top;
+}
+
+static qnode *node_new(const qnode *parent, int B, uint64_t id)
+{
+    qnode *x = malloc(sizeof(qnode) + (size_t)B * (2 * sizeof(int) + sizeof(double)));
+    if (!x) { bqb_set_error("node allocation failed"); return NULL; }
+    x->fracsol = (double *)(x + 1);
+    x->xfixed = (int *)(x->fracsol + B);
+    x->solX = x->xfixed + B;
+    x->id = id;
+    memset(x->fracsol, 0, sizeof(double) * B);
+    if (parent) {
+        for (int i = 0; i < B; ++i) {
+            x->xfixed[i] = parent->xfixed[i];
+            x->solX[i] = parent->xfixed[i] ? parent->solX[i] : 0;
+        }
+        x->level = parent->level + 1;
+        x->upper_bound = parent->upper_bound;
+    } else {
+        memset(x->xfixed, 0, sizeof(int) * B);
+        memset(x->solX, 0, sizeof(int) * B);
+        x->level = 0;
+        x->upper_bound = 0.0;
+    }
+    return x;
+}
+
+/* bab_functions.c:387-421 */
+static int branching_variable(const qnode *x, int B, int strategy)
+{
+    int ic = -1;
+    if (strategy == 0) {             /* LEAST_FRACTIONAL */
+        double best = -1e9;
+        for (int i = 0; i < B; ++i)
+            if (!x->xfixed[i] && fabs(0.5 - x->fracsol[i]) > best) { ic = i; best = fabs(0.5 - x->fracsol[i]); }
+    } else {                         /* MOST_FRACTIONAL */
+        double best = 1e9;
+        for (int i = 0; i < B; ++i)
+            if (!x->xfixed[i] && fabs(0.5 - x->fracsol[i]) < best) { ic = i; best = fabs(0.5 - x->fracsol[i]); }
+    }
+    return ic;
+}
+
+static double now_s(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
+              int *sol, bqb_solve_stats *out)
+{
+    memset(out, 0, sizeof *out);
+    const double t0 = now_s();
+    if (batch < 1) batch = 1;
+    const int B = n0 - 1;
+    bqb_ctx *ctx = bqb_ctx_create(n0, SP_L, params, batch, 0, 0, 0, 0);
+    if (!ctx) return -1;
+    qheap h = {0};
+    int rc = -1;
+    uint64_t next_id = 0;
+    double LB = 0.0;                       /* bab_functions.c:26-35 */
+    memset(sol, 0, sizeof(int) * B);
+    double diff = 0.0;
+    qnode **cur = calloc(batch, sizeof(qnode *));
+    int *xf = malloc(sizeof(int) * (size_t)batch * B);
+    int *sx = malloc(sizeof(int) * (size_t)batch * B);
+    double *lb = malloc(sizeof(double) * batch), *df = malloc(sizeof(double) * batch);
+    int *rt = malloc(sizeof(int) * batch);
+    uint64_t *sd = malloc(sizeof(uint64_t) * batch);
+    double *bound = malloc(sizeof(double) * batch), *frac = malloc(sizeof(double) * (size_t)batch * B);
+    int *xb = malloc(sizeof(int) * (size_t)batch * B);
+    double *lbo = malloc(sizeof(double) * batch), *dfo = malloc(sizeof(double) * batch);
+    int *imp = malloc(sizeof(int) * batch), *st = malloc(sizeof(int) * batch);
+    if (!cur || !xf || !sx || !lb || !df || !rt || !sd || !bound || !frac || !xb || !lbo || !dfo || !imp || !st) {
+        bqb_set_error("calloc");
+        goto done;
+    }
+
+    /* root node (bab_functions.c:39-69) */
+    {
+        qnode *root = node_new(NULL, B, next_id++);
+        if (!root) goto done;
+        memcpy(xf, root->xfixed, sizeof(int) * B);
+        memcpy(sx, root->solX, sizeof(int) * B);
+        lb[0] = LB; df[0] = 0.0; rt[0] = 1; sd[0] = bqb_key(seed, root->id);
+        if (bqb_bound_batch(ctx, 1, xf, sx, lb, df, rt, sd, bound, frac, xb, lbo, imp, dfo, st, NULL)) {
+            free(root);
+            goto done;
+        }
+        out->bound_ms += bqb_ctx_last_ms(ctx);
+        out->nodes = 1;
+        out->batches = 1;
+        if (st[0]) {
+            out->status = st[0];
+            bqb_set_error("root node bound failed with status %d", st[0]);
+            free(root);
+            goto done;
+        }
+        if (imp[0] && lbo[0] > LB) { LB = lbo[0]; memcpy(sol, xb, sizeof(int) * B); }
+        diff = dfo[0];
+        root->upper_bound = bound[0];
+        out->root_bound = bound[0];
+        memcpy(root->fracsol, frac, sizeof(double) * B);
+        if (LB + 1.0 < root->upper_bound && !params->root) {
+            const int ic = branching_variable(root, B, params->branchingStrategy);
+            for (int v = 0; v <= 1 && ic >= 0; ++v) {
+                qnode *c = node_new(root, B, next_id++);
+                if (!c || heap_push(&h, c)) { free(c); free(root); goto done; }
+                c->xfixed[ic] = 1;
+                c->solX[ic] = v;
+            }
+        }
+        free(root);
+    }
+
+    while (h.used > 0) {
+        if (params->time_limit > 0 && now_s() - t0 > params->time_limit) { out->stopped = 1; break; }
+        if (h.used > out->max_queue) out->max_queue = (int)h.used;
+        int nb = 0;
+        while (nb < batch && h.used > 0) {
+            qnode *x = heap_pop(&h);
+            if (LB + 1.0 < x->upper_bound) cur[nb++] = x;
+            else free(x);                 /* cannot beat the incumbent any more */
+        }
+        if (!nb) break;
+        for (int b = 0; b < nb; ++b) {
+            memcpy(xf + (size_t)b * B, cur[b]->xfixed, sizeof(int) * B);
+            memcpy(sx + (size_t)b * B, cur[b]->solX, sizeof(int) * B);
+            lb[b] = LB; df[b] = diff; rt[b] = 0; sd[b] = bqb_key(seed, cur[b]->id);
+        }
+        if (bqb_bound_batch(ctx, nb, xf, sx, lb, df, rt, sd, bound, frac, xb, lbo, imp, dfo, st, NULL)) {
+            for (int b = 0; b < nb; ++b) free(cur[b]);
+            goto done;
+        }
+        out->bound_ms += bqb_ctx_last_ms(ctx);
+        out->nodes += nb;
+        out->batches += 1;
+        for (int b = 0; b < nb; ++b) {
+            if (st[b] && !out->status) {
+                out->status = st[b];
+                bqb_set_error("node bound failed with status %d", st[b]);
+            }
+            if (imp[b] && lbo[b] > LB) { LB = lbo[b]; memcpy(sol, xb + (size_t)b * B, sizeof(int) * B); }
+            cur[b]->upper_bound = bound[b];
+            memcpy(cur[b]->fracsol, frac + (size_t)b * B, sizeof(double) * B);
+        }
+        int fail = out->status != 0;
+        for (int b = 0; b < nb; ++b) {
+            qnode *x = cur[b];
+            if (!fail && LB + 1.0 < x->upper_bound) {
+                const int ic = branching_variable(x, B, params->branchingStrategy);
+                for (int v = 0; v <= 1 && ic >= 0; ++v) {
+                    qnode *c = node_new(x, B, next_id++);
+                    if (!c || heap_push(&h, c)) { free(c); fail = 1; break; }
+                    c->xfixed[ic] = 1;
+                    c->solX[ic] = v;
+                }
+            }
+            free(x);
+        }
+        if (fail) goto done;
+    }
+    rc = 0;
+done:
+    while (h.used > 0) free(heap_pop(&h));
+    free(h.d);
+    free(cur); free(xf); free(sx); free(lb); free(df); free(rt); free(sd); free(bound);
+    free(frac); free(xb); free(lbo); free(dfo); free(imp); free(st);
+    bqb_ctx_destroy(ctx);
+    out->best = LB;
+    out->seconds = now_s() - t0;
+    return rc;
+}

@@ -356,6 +356,8 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
         args.iters[node] = iter;
         args.trials[node] = trials;
         args.status[node] = status;
+        if (args.flops)
+            atomicAdd(args.flops, (unsigned long long)BQB_IPM_FLOPS(n, iter, trials));
     }
     if (args.y_out)
         for (int I = t.tid; I < n; I += P * P) args.y_out[(size_t)node * args.ystride + I] = s_y[I];

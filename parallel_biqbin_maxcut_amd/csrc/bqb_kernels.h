@@ -47,7 +47,14 @@ typedef struct IpmBatchArgs {
     int trace_cap;
     int max_iters;         /* 0: default cap; k > 0: stop after k iterations */
     double *dbg;           /* optional [nodes * 4 * ld]: initial y, zd, diag(L), and phi/psi/mu */
+    unsigned long long *flops;  /* optional: += algorithmic flops of each node's solve */
 } IpmBatchArgs;
+
+/* Algorithmic FP64 flops of one IPM solve of size n: per iteration two SPD
+ * inversions (n^3 each) and two n x n products (2 n^3 each), plus n^3/3 per
+ * Cholesky test of the line searches. */
+#define BQB_IPM_FLOPS(n, iters, trials) \
+    ((double)(n) * (n) * (n) * (6.0 * (iters) + (trials) / 3.0))
 
 int bqb_ipm_tile_for(int nmax);
 hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int tile, hipStream_t s);

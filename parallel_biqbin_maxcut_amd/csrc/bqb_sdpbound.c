@@ -47,10 +47,12 @@ struct bqb_ctx {
     hipEvent_t ev0, ev1;
     NodeState *h_st;
     double last_ms;
-    double ipm_ms;        /* accumulated time of the IPM launches (events) */
-    long long ipm_iters;  /* sum of IPM iterations over all IPM calls */
-    long long ipm_flops;  /* algorithmic IPM flops of the last batch */
+    /* IPM launch profile: an event pair around every IPM launch while
+     * profiling is on, and a device counter of algorithmic IPM flops */
     int profile_ipm;
+    hipEvent_t *pev;
+    int pev_cap, pev_n;
+    unsigned long long *d_flops;
 };
 
 #define CK(call, what) do { if (bqb_check((call), (what))) goto fail; } while (0)
@@ -149,6 +151,7 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         {(void **)&c->p.sa_val, sizeof(double) * N * BQB_SA_CHAINS},
         {(void **)&c->p.sa_perm, sizeof(int) * N * BQB_SA_CHAINS * 7},
         {(void **)&c->p.sa_status, sizeof(int) * N * BQB_SA_CHAINS},
+        {(void **)&c->d_flops, sizeof(unsigned long long)},
     };
     const int nitems = (int)(sizeof items / sizeof items[0]);
     size_t total = 0;
@@ -178,10 +181,19 @@ fail:
     return NULL;
 }
 
+static void free_events(bqb_ctx *c)
+{
+    for (int i = 0; i < c->pev_cap; ++i) hipEventDestroy(c->pev[i]);
+    free(c->pev);
+    c->pev = NULL;
+    c->pev_cap = c->pev_n = 0;
+}
+
 void bqb_ctx_destroy(bqb_ctx *c)
 {
     if (!c) return;
     hipStreamSynchronize(c->stream);
+    free_events(c);
     hipFree(c->arena);
     hipStreamDestroy(c->stream);
     hipEventDestroy(c->ev0);
@@ -208,14 +220,15 @@ static int ipm_launch(bqb_ctx *c, const double *L, double *X)
     a.trials = c->p.ipm_trials;
     a.status = c->p.ipm_status;
     a.active = c->p.mask;
-    if (c->profile_ipm) hipEventRecord(c->ev0, c->stream);
+    const int prof = c->profile_ipm && c->pev_n + 2 <= c->pev_cap;
+    if (prof) {
+        a.flops = c->d_flops;
+        hipEventRecord(c->pev[c->pev_n], c->stream);
+    }
     if (bqb_check(bqb_launch_ipm_batch(&a, c->d.nodes, c->tile, c->stream), "ipm launch")) return -1;
-    if (c->profile_ipm) {
-        float ms = 0.f;
-        hipEventRecord(c->ev1, c->stream);
-        hipEventSynchronize(c->ev1);
-        hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        c->ipm_ms += ms;
+    if (prof) {
+        hipEventRecord(c->pev[c->pev_n + 1], c->stream);
+        c->pev_n += 2;
     }
     return 0;
 }
@@ -426,5 +439,42 @@ int bqb_ctx_upload(bqb_ctx *c, int id, int node, const void *host, size_t bytes)
 
 int bqb_ctx_sync(bqb_ctx *c) { return bqb_check(hipStreamSynchronize(c->stream), "sync"); }
 
-void bqb_ctx_profile_ipm(bqb_ctx *c, int on) { c->profile_ipm = on; c->ipm_ms = 0.0; }
-double bqb_ctx_ipm_ms(const bqb_ctx *c) { return c->ipm_ms; }
+/* Start (max_launches > 0) or stop (0) timing every IPM launch with an event
+ * pair and counting its algorithmic flops on the device.  Starting resets
+ * the counters. */
+int bqb_ctx_profile_ipm(bqb_ctx *c, int max_launches)
+{
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (max_launches <= 0) { c->profile_ipm = 0; return 0; }
+    if (2 * max_launches > c->pev_cap) {
+        free_events(c);
+        c->pev = calloc((size_t)2 * max_launches, sizeof(hipEvent_t));
+        if (!c->pev) { bqb_set_error("calloc"); return -1; }
+        for (int i = 0; i < 2 * max_launches; ++i) {
+            if (bqb_check(hipEventCreate(&c->pev[i]), "hipEventCreate")) { c->pev_cap = i; return -1; }
+        }
+        c->pev_cap = 2 * max_launches;
+    }
+    c->pev_n = 0;
+    c->profile_ipm = 1;
+    return bqb_check(hipMemset(c->d_flops, 0, sizeof(unsigned long long)), "hipMemset flops");
+}
+
+/* IPM launches timed since profiling started: count, summed duration (ms)
+ * and algorithmic flops. */
+int bqb_ctx_ipm_stats(bqb_ctx *c, int *launches, double *ms_total, double *flops)
+{
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    double ms = 0.0;
+    for (int i = 0; i + 1 < c->pev_n; i += 2) {
+        float e = 0.f;
+        if (bqb_check(hipEventElapsedTime(&e, c->pev[i], c->pev[i + 1]), "hipEventElapsedTime")) return -1;
+        ms += e;
+    }
+    unsigned long long f = 0;
+    if (bqb_check(hipMemcpy(&f, c->d_flops, sizeof f, hipMemcpyDeviceToHost), "D2H flops")) return -1;
+    if (launches) *launches = c->pev_n / 2;
+    if (ms_total) *ms_total = ms;
+    if (flops) *flops = (double)f;
+    return 0;
+}

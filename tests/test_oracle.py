@@ -11,35 +11,8 @@ import numpy as np
 import pytest
 
 import oracle_py as O
+from fixture_util import brute_maxcut, fixture_solution, fixture_value, laplacian
 from parallel_biqbin_maxcut_amd import instances as I
-
-
-def laplacian(adj):
-    return np.diag(adj.sum(1)) - adj
-
-
-def brute_maxcut(adj):
-    n = adj.shape[0]
-    best = -1e18
-    for bits in range(1 << (n - 1)):       # last vertex fixed to side 0
-        x = np.array([(bits >> i) & 1 for i in range(n - 1)] + [0])
-        s = 2 * x - 1
-        best = max(best, 0.25 * s @ laplacian(adj) @ s)
-    return best
-
-
-def fixture_solution(path):
-    for line in open(path):
-        if line.startswith("Solution"):
-            return [int(t) for t in line.split("(")[1].split(")")[0].split()]
-        if line.startswith("Maximum value"):
-            pass
-
-
-def fixture_value(path):
-    for line in open(path):
-        if line.startswith("Maximum value"):
-            return float(line.split("=")[1])
 
 
 # ---------------------------------------------------------------- IPM

@@ -107,6 +107,10 @@ double SDPbound(BqbBabNode *node, BqbProblem *SP, BqbProblem *PP, int rank);
 void createSubproblem(BqbBabNode *node, BqbProblem *SP, BqbProblem *PP);
 /* replaces src/evaluate.c:123 (src/biqbin.h:157) */
 double getFixedValue(BqbBabNode *node, BqbProblem *SP);
+/* replaces src/bundle.c:242 (src/biqbin.h:140); reads the host's global cut
+ * lists Cuts / Pent_Cuts / Hepta_Cuts (src/global_var.h:42-51) with the
+ * counts in PP */
+double fct_eval(const BqbProblem *PP, double *dual_gamma, double *X, double *g);
 #endif
 /* The library-internal incumbent of the drop-in entry points (value; sol
  * receives n0-1 entries if not NULL) and release of their device context. */
@@ -214,6 +218,16 @@ int bqb_ctx_sync(bqb_ctx *c);
  * flops (6 n^3 per iteration + n^3/3 per line-search Cholesky test). */
 int bqb_ctx_profile_ipm(bqb_ctx *c, int max_launches);
 int bqb_ctx_ipm_stats(bqb_ctx *c, int *launches, double *ms_total, double *flops);
+
+/* fct_eval (src/bundle.c:242-275) on explicit cut lists, using node slot 0
+ * of a context whose tile fits n: X = IPM solution for L - B'(gamma)
+ * (n x n column-major in and out; X may be NULL), g = 1 - B(X) (m entries,
+ * may be NULL), *f = phi + sum(gamma).  Cut records: tri 4 ints (i, j, k,
+ * type 1-4), pent 6 ints (type 1-3, 5 vertices), hept 8 ints (type 1-4, 7
+ * vertices), vertices 0-based.  *status receives the IPM status. */
+int bqb_fct_eval(bqb_ctx *c, int n, const double *L, int ntri, const int *tri, int npent,
+                 const int *pent, int nhept, const int *hept, const double *gamma, double *X,
+                 double *g, double *f, int *status);
 
 /* Diagnostic: one matrix, stop after max_iters iterations and return the
  * kernel's internal inv(Z) (Zi), inv(Zi o X) (Mi) and search direction dX of

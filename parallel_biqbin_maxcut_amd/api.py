@@ -181,6 +181,8 @@ def _declare_host(L):
     L.createSubproblem.restype = None
     L.getFixedValue.argtypes = [C.POINTER(BabNode), C.POINTER(Problem)]
     L.getFixedValue.restype = C.c_double
+    L.fct_eval.argtypes = [C.POINTER(Problem), N.dptr, N.dptr, N.dptr]
+    L.fct_eval.restype = C.c_double
     L.bqb_dropin_incumbent.argtypes = [N.iptr]
     L.bqb_dropin_incumbent.restype = C.c_double
     L.bqb_dropin_release.argtypes = []
@@ -255,6 +257,8 @@ def _declare_ctx(L):
     u64p = C.POINTER(C.c_uint64)
     L.bqb_bound_batch.argtypes = [vp, C.c_int, N.iptr, N.iptr, N.dptr, N.dptr, N.iptr, u64p, N.dptr, N.dptr,
                                   N.iptr, N.dptr, N.iptr, N.dptr, N.iptr, N.iptr]
+    L.bqb_fct_eval.argtypes = [vp, C.c_int, N.dptr, C.c_int, N.iptr, C.c_int, N.iptr, C.c_int, N.iptr, N.dptr,
+                               N.dptr, N.dptr, N.dptr, N.iptr]
     L.bqb_ctx_load.argtypes = [vp, C.c_int, N.iptr, N.iptr, N.dptr, N.dptr, N.iptr, u64p]
     L.bqb_ctx_run.argtypes = [vp, C.c_int, C.c_int]
     L.bqb_ctx_bound.argtypes = [vp]
@@ -343,6 +347,25 @@ class BoundContext:
 
     def set_state(self, node, s):
         N.check(N.lib().bqb_ctx_upload(self.h, 0, node, C.addressof(s), C.sizeof(s)))
+
+    def fct_eval(self, L, tris=(), pents=(), hepts=(), gamma=()):
+        """fct_eval (src/bundle.c:242-275) of matrix L with the given cuts
+        (tri: (i, j, k, type); pent/hept: (type, vertices)).  Returns
+        (f, X, g, status)."""
+        L = np.asfortranarray(L, dtype=np.float64)
+        n = L.shape[0]
+        t = np.array([list(c) for c in tris] or [[0, 0, 0, 0]], dtype=np.int32)
+        p = np.array([[c[0]] + list(c[1]) for c in pents] or [[0] * 6], dtype=np.int32)
+        h = np.array([[c[0]] + list(c[1]) for c in hepts] or [[0] * 8], dtype=np.int32)
+        m = len(tris) + len(pents) + len(hepts)
+        gam = np.ascontiguousarray(np.asarray(gamma, dtype=np.float64).reshape(-1)[:m] if m else np.zeros(1))
+        X = np.zeros((n, n), order="F")
+        g = np.zeros(max(m, 1))
+        f = C.c_double()
+        st = C.c_int()
+        N.check(N.lib().bqb_fct_eval(self.h, n, _d(L), len(tris), _i(t), len(pents), _i(p), len(hepts), _i(h),
+                                     _d(gam), _d(X), _d(g), C.byref(f), C.byref(st)))
+        return f.value, X, g[:m], st.value
 
     def sync(self):
         N.check(N.lib().bqb_ctx_sync(self.h))

@@ -373,6 +373,22 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         return;
     }
 
+    if (phase == PH_FCT_FINISH) {
+        // bundle.c:242-275 after the IPM on L0: f = phi + gamma_1 + ... in
+        // order (one lane, so it rounds as the reference's loop), g = 1 - B(X)
+        if (!p.mask[node]) return;
+        const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
+        subgradient(d, p, node, s, p.Xt + node * slab, p.g + (size_t)node * d.max_m);
+        if (threadIdx.x == 0) {
+            const double *gam = p.gamma + (size_t)node * d.max_m;
+            double f = p.phi[node];
+            for (int c = 0; c < m; ++c) f += gam[c];
+            sp->f_test = f;
+            sp->status = p.ipm_status[node];
+        }
+        return;
+    }
+
     if (phase == PH_FINISH) {
         if (threadIdx.x == 0) {
             if (sp->active) { sp->active = 0; sp->bound = s.f + s.fixedvalue; }

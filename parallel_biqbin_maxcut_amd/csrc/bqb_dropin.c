@@ -31,6 +31,15 @@ extern int updateSolution(int *x) __attribute__((weak));
 extern BiqBinParameters params __attribute__((weak));
 extern double diff __attribute__((weak));
 
+/* Cut records of the reference (src/biqbin.h:81-107) and its global cut
+ * lists (src/global_var.h:42-51), read by fct_eval. */
+typedef struct RefTri { int i, j, k, type; double value, y; } RefTri;
+typedef struct RefPent { int type; int perm[5]; double value, y; } RefPent;
+typedef struct RefHept { int type; int perm[7]; double value, y; } RefHept;
+extern RefTri *Cuts __attribute__((weak));
+extern RefPent *Pent_Cuts __attribute__((weak));
+extern RefHept *Hepta_Cuts __attribute__((weak));
+
 static struct {
     bqb_ctx *ctx;
     const double *SP_L;
@@ -163,6 +172,60 @@ double Evaluate(BqbBabNode *node, BqbProblem *SP, BqbProblem *PP, int rank)
     return SDPbound(node, SP, PP, rank);
 }
 
+/* fct_eval's own context: any matrix up to its tile, cut capacities grown
+ * on demand. */
+static struct { bqb_ctx *ctx; int n0, tri, pent, hept; } g_fct;
+
+static bqb_ctx *fct_context(int n, int ntri, int npent, int nhept)
+{
+    if (g_fct.ctx && bqb_ctx_ld(g_fct.ctx) >= n && ntri <= g_fct.tri && npent <= g_fct.pent && nhept <= g_fct.hept)
+        return g_fct.ctx;
+    if (g_fct.ctx) bqb_ctx_destroy(g_fct.ctx);
+    g_fct.ctx = NULL;
+    const int n0 = n > g_fct.n0 ? n : g_fct.n0;
+    g_fct.tri = ntri > 8192 ? ntri : 8192;
+    g_fct.pent = npent > 2048 ? npent : 2048;
+    g_fct.hept = nhept > 2048 ? nhept : 2048;
+    double *zero = calloc((size_t)n0 * n0, sizeof(double));
+    BiqBinParameters p;
+    bqb_params_default(&p);
+    if (zero) g_fct.ctx = bqb_ctx_create(n0, zero, &p, 1, 0, g_fct.tri, g_fct.pent, g_fct.hept);
+    free(zero);
+    g_fct.n0 = n0;
+    return g_fct.ctx;
+}
+
+/* src/bundle.c:242-275 with the host's global cut lists (Cuts, Pent_Cuts,
+ * Hepta_Cuts, counts in PP): X = IPM solution for PP->L - B'(gamma),
+ * g = 1 - B(X), returns phi + sum(gamma). */
+double fct_eval(const BqbProblem *PP, double *dual_gamma, double *X, double *g)
+{
+    const int nt = PP->NIneq, np = PP->NPentIneq, nh = PP->NHeptaIneq;
+    if ((nt && !(&Cuts && Cuts)) || (np && !(&Pent_Cuts && Pent_Cuts)) || (nh && !(&Hepta_Cuts && Hepta_Cuts))) {
+        fprintf(stderr, "Error: fct_eval needs the host's cut lists (Cuts, Pent_Cuts, Hepta_Cuts)\n");
+        exit(10);
+    }
+    int *tri = malloc(sizeof(int) * 4 * (nt ? nt : 1));
+    int *pent = malloc(sizeof(int) * 6 * (np ? np : 1));
+    int *hept = malloc(sizeof(int) * 8 * (nh ? nh : 1));
+    if (!tri || !pent || !hept) { fprintf(stderr, "Error: memory allocation in fct_eval\n"); exit(10); }
+    for (int q = 0; q < nt; ++q) {
+        tri[4 * q] = Cuts[q].i; tri[4 * q + 1] = Cuts[q].j; tri[4 * q + 2] = Cuts[q].k; tri[4 * q + 3] = Cuts[q].type;
+    }
+    for (int q = 0; q < np; ++q) { pent[6 * q] = Pent_Cuts[q].type; memcpy(pent + 6 * q + 1, Pent_Cuts[q].perm, 5 * sizeof(int)); }
+    for (int q = 0; q < nh; ++q) { hept[8 * q] = Hepta_Cuts[q].type; memcpy(hept + 8 * q + 1, Hepta_Cuts[q].perm, 7 * sizeof(int)); }
+    bqb_ctx *c = fct_context(PP->n, nt, np, nh);
+    double f = 0.0;
+    int status = 0;
+    if (!c || bqb_fct_eval(c, PP->n, PP->L, nt, tri, np, pent, nh, hept, dual_gamma, X, g, &f, &status)) {
+        fprintf(stderr, "Error: fct_eval: %s\n", bqb_last_error());
+        exit(10);
+    }
+    if (status) { fprintf(stderr, "Error: fct_eval: IPM status %d\n", status); exit(10); }
+    free(tri); free(pent); free(hept);
+    return f;
+}
+
 /* The library-internal incumbent (hosts without Bab_LBGet/updateSolution). */
 double bqb_dropin_incumbent(int *sol)
 {
@@ -175,4 +238,6 @@ void bqb_dropin_release(void)
 {
     if (g_drop.ctx) bqb_ctx_destroy(g_drop.ctx);
     memset(&g_drop, 0, sizeof g_drop);
+    if (g_fct.ctx) bqb_ctx_destroy(g_fct.ctx);
+    memset(&g_fct, 0, sizeof g_fct);
 }

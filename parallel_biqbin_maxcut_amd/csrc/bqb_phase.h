@@ -13,6 +13,7 @@ enum {
     PH_SEP_HEPT = 8,     /* heptagonal separation decision */
     PH_OUTER_C = 9,      /* stop test, fracsol, gamma from the cuts */
     PH_REBUILD = 10,     /* extend the bundle after fct_eval(gamma) */
-    PH_FINISH = 11
+    PH_FINISH = 11,
+    PH_FCT_FINISH = 12   /* fct_eval tail: f = phi + sum(gamma), g = 1 - B(X_test) */
 };
 #endif

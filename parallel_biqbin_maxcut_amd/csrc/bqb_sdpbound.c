@@ -251,7 +251,7 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     case 108: return bqb_check(bqb_launch_bundle_qp(d, p, s), "qp launch");
     case 109: return bqb_check(bqb_launch_bundle_step(d, p, s), "step launch");
     default:
-        if (what >= 0 && what <= PH_FINISH)
+        if (what >= 0 && what <= PH_FCT_FINISH)
             return bqb_check(bqb_launch_control(d, p, what, arg, s), "control launch");
         bqb_set_error("unknown phase %d", what);
         return -1;
@@ -375,6 +375,85 @@ int bqb_bound_batch(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, c
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->last_ms = ms;
     return bqb_ctx_results(c, bound, fracsol, xbest, LB_out, improved, diff_out, status, stats);
+}
+
+/* fct_eval (src/bundle.c:242-275) of one matrix on node slot 0:
+ * X = argmax of the basic SDP for L - B'(gamma), f = phi + sum(gamma),
+ * g = 1 - B(X).  Cut records: tri 4 ints (i, j, k, type), pent 6 ints
+ * (type, 5 vertices), hept 8 ints (type, 7 vertices); vertices 0-based < n.
+ * L and X are n x n column-major (X may be NULL), g has m entries (may be
+ * NULL).  Returns 0, or -1 with bqb_last_error(); *status receives the IPM
+ * status. */
+int bqb_fct_eval(bqb_ctx *c, int n, const double *L, int ntri, const int *tri, int npent,
+                 const int *pent, int nhept, const int *hept, const double *gamma, double *X,
+                 double *g, double *f, int *status)
+{
+    const BatchDims d0 = c->d;
+    const int ld = d0.ld, m = ntri + npent + nhept;
+    if (n < 1 || n > ld) { bqb_set_error("fct_eval: n = %d outside 1..%d", n, ld); return -1; }
+    if (ntri < 0 || npent < 0 || nhept < 0 || ntri > d0.max_tri || npent > d0.max_pent || nhept > d0.max_hept) {
+        bqb_set_error("fct_eval: cut counts exceed the context capacity");
+        return -1;
+    }
+    const int *lists[3] = {tri, pent, hept};
+    const int counts[3] = {ntri, npent, nhept}, widths[3] = {3, 5, 7}, stride[3] = {4, 6, 8};
+    for (int t = 0; t < 3; ++t)
+        for (int q = 0; q < counts[t]; ++q) {
+            const int *r = lists[t] + (size_t)q * stride[t];
+            const int *v = (t == 0) ? r : r + 1;
+            const int type = (t == 0) ? r[3] : r[0];
+            if (type < 1 || type > (t == 1 ? 3 : 4)) { bqb_set_error("fct_eval: bad cut type"); return -1; }
+            for (int a = 0; a < widths[t]; ++a) {
+                if (v[a] < 0 || v[a] >= n) { bqb_set_error("fct_eval: cut vertex out of range"); return -1; }
+                for (int b = 0; b < a; ++b)
+                    if (v[a] == v[b]) { bqb_set_error("fct_eval: repeated cut vertex"); return -1; }
+            }
+        }
+    const size_t slab = (size_t)ld * ld;
+    double *hL = calloc(slab, sizeof(double));
+    TriCut *ht = malloc(sizeof(TriCut) * (ntri ? ntri : 1));
+    KgonCut *hp = calloc(npent ? npent : 1, sizeof(KgonCut));
+    KgonCut *hh = calloc(nhept ? nhept : 1, sizeof(KgonCut));
+    int rc = -1;
+    if (!hL || !ht || !hp || !hh) { bqb_set_error("calloc"); goto out; }
+    for (int j = 0; j < n; ++j) memcpy(hL + (size_t)j * ld, L + (size_t)j * n, sizeof(double) * n);
+    for (int q = 0; q < ntri; ++q) {
+        ht[q].i = tri[4 * q]; ht[q].j = tri[4 * q + 1]; ht[q].k = tri[4 * q + 2]; ht[q].type = tri[4 * q + 3];
+    }
+    for (int q = 0; q < npent; ++q) { hp[q].type = pent[6 * q]; memcpy(hp[q].p, pent + 6 * q + 1, 5 * sizeof(int)); }
+    for (int q = 0; q < nhept; ++q) { hh[q].type = hept[8 * q]; memcpy(hh[q].p, hept + 8 * q + 1, 7 * sizeof(int)); }
+    NodeState st;
+    memset(&st, 0, sizeof st);
+    st.n = n; st.active = 1; st.NIneq = ntri; st.NPentIneq = npent; st.NHeptaIneq = nhept;
+    const int one = 1;
+    c->d.nodes = 1;
+    hipStream_t s = c->stream;
+#define UP(dst, src, bytes, what) if (bqb_check(hipMemcpyAsync((void *)(dst), (src), (bytes), hipMemcpyHostToDevice, s), what)) goto out
+    UP(c->p.L, hL, sizeof(double) * slab, "H2D L");
+    UP(c->p.st, &st, sizeof st, "H2D state");
+    UP(c->p.nvec, &n, sizeof(int), "H2D n");
+    UP(c->p.mask, &one, sizeof(int), "H2D mask");
+    if (ntri) UP(c->p.tri, ht, sizeof(TriCut) * ntri, "H2D cuts");
+    if (npent) UP(c->p.pent, hp, sizeof(KgonCut) * npent, "H2D cuts");
+    if (nhept) UP(c->p.hept, hh, sizeof(KgonCut) * nhept, "H2D cuts");
+    if (m) UP(c->p.gamma, gamma, sizeof(double) * m, "H2D gamma");
+#undef UP
+    if (bqb_ctx_run(c, 106, 0) || bqb_ctx_run(c, 107, 0) || bqb_ctx_run(c, 101, 0) ||
+        bqb_ctx_run(c, PH_FCT_FINISH, 0))
+        goto out;
+    if (bqb_check(hipMemcpyAsync(&st, c->p.st, sizeof st, hipMemcpyDeviceToHost, s), "D2H state")) goto out;
+    if (X && bqb_check(hipMemcpyAsync(hL, c->p.Xt, sizeof(double) * slab, hipMemcpyDeviceToHost, s), "D2H X")) goto out;
+    if (g && m && bqb_check(hipMemcpyAsync(g, c->p.g, sizeof(double) * m, hipMemcpyDeviceToHost, s), "D2H g")) goto out;
+    if (bqb_check(hipStreamSynchronize(s), "sync")) goto out;
+    if (X)
+        for (int j = 0; j < n; ++j) memcpy(X + (size_t)j * n, hL + (size_t)j * ld, sizeof(double) * n);
+    if (f) *f = st.f_test;
+    if (status) *status = st.status;
+    rc = 0;
+out:
+    c->d.nodes = d0.nodes;
+    free(hL); free(ht); free(hp); free(hh);
+    return rc;
 }
 
 /* ---- raw access to one node's device buffers (tests, diagnostics) ---- */

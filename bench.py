@@ -8,9 +8,13 @@ full SDPbound (FP64 IPM + bundle over triangle / pentagonal / heptagonal cuts,
 GW heuristic).  Every node receives the incumbent and the root gap of the
 instance's root bound, as a B&B worker does.
 
-One step = one bqb_bound_batch call over the 512 nodes (inputs uploaded inside
-the call, a few hundred KB; the timed region includes it).  value = node
-bounds per second.  The roofline object covers the dominant kernel, the batched
+One step = 2048 such nodes bounded through a context of 512 node slots
+(bqb_bound_stream: every launch covers the 512 slots, a finished node's slot
+is refilled after each outer round, as the B&B node queue runs it).  Inputs
+are uploaded inside the call (a few hundred KB; the timed region includes
+it).  value = node bounds per second.  The same nodes bounded as one fixed
+512-node batch (every launch waits for the slowest node) are reported in
+config.single_batch_bounds_per_s.  The roofline object covers the dominant kernel, the batched
 IPM: algorithmic FP64 flops (6 n^3 per iteration + n^3/3 per line-search
 Cholesky test, counted on the device) over the summed duration of its launches
 (HIP event pairs on the launch stream, during the timed steps).  The
@@ -60,7 +64,7 @@ def cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample):
     bounds, status = O.sdpbound_many(SP, op, xf[:sample], sx[:sample], LB, diff, seeds[:sample], cores)
     dt = time.perf_counter() - t0
     return {"value": sample / dt, "unit": "bounds/s", "cores": cores, "kind": "port",
-            "sample": f"first {sample} of the {len(xf)} nodes (same instance, incumbent and root gap), "
+            "sample": f"first {sample} of the {len(xf)} nodes of a step (same instance, incumbent and root gap), "
                       f"{dt:.1f} s wall on {cores} threads"}, bounds, status
 
 
@@ -79,7 +83,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nodes", type=int, default=512)
+    ap.add_argument("--nodes", type=int, default=512, help="node slots (subproblems per launch)")
+    ap.add_argument("--per-step", type=int, default=2048, help="nodes bounded per step")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -91,7 +96,7 @@ def main():
 
     from parallel_biqbin_maxcut_amd import api, instances as I
 
-    SP, P, xf, sx, seeds = build_workload(api, I, args.seed, args.nodes)
+    SP, P, xf, sx, seeds = build_workload(api, I, args.seed, args.per_step)
     ctx = api.BoundContext(SP, params=P, max_nodes=args.nodes)
     # root bound of the instance: incumbent and root gap the workers receive
     z = np.zeros((1, N0 - 1), np.int32)
@@ -101,12 +106,12 @@ def main():
     LB, diff = float(root["LB"][0]), float(root["diff"][0])
 
     for _ in range(args.warmup):
-        ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds)
-    ctx.profile_ipm(max_launches=2000 * max(1, args.steps))
+        ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
+    ctx.profile_ipm(max_launches=5000 * max(1, args.steps))
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r = ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds)
+        r = ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
     ctx.sync()
     dt = time.perf_counter() - t0
     launches, ipm_ms, flops = ctx.ipm_stats()
@@ -114,8 +119,15 @@ def main():
     if np.any(r["status"]):
         raise SystemExit(f"node bound failures: {np.unique(r['status'], return_counts=True)}")
 
+    # the first 512 nodes as one fixed batch (all launches wait for its slowest node)
+    tb = time.perf_counter()
+    rb = ctx.bound(xf[: args.nodes], sx[: args.nodes], LB=LB, diff=diff, is_root=0, seeds=seeds[: args.nodes])
+    single_batch = args.nodes / (time.perf_counter() - tb)
+    if not np.array_equal(rb["bound"], r["bound"][: args.nodes]):
+        raise SystemExit("streamed and batched bounds differ")
+
     ms_per_step = dt / args.steps * 1e3
-    value = args.nodes * args.steps / dt
+    value = args.per_step * args.steps / dt
     achieved = flops / (ipm_ms / 1e3) / 1e12 if ipm_ms > 0 else 0.0
     stats = r["stats"]
     line = {
@@ -136,6 +148,8 @@ def main():
                         "per launch with 0-40 variables fixed at random (sizes 60-100), FP64 IPM + bundle "
                         "with triangle/pentagonal/heptagonal cuts",
             "nodes_per_launch": args.nodes,
+            "nodes_per_step": args.per_step,
+            "single_batch_bounds_per_s": round(single_batch, 2),
             "seed": args.seed,
             "incumbent": LB,
             "root_gap": round(diff, 4),
@@ -158,7 +172,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        sample = min(args.cpu_sample, args.nodes)
+        sample = min(args.cpu_sample, args.per_step)
         cb, cbounds, cstatus = cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample)
         line["cpu_baseline"] = cb
         ok = cstatus == 0

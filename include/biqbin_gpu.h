@@ -182,23 +182,44 @@ size_t bqb_ctx_bytes(const bqb_ctx *c);
 int bqb_ctx_ld(const bqb_ctx *c);
 double bqb_ctx_last_ms(const bqb_ctx *c);
 
-/* Bounds `nodes` B&B nodes with the algorithm of src/bounding.c:24-342
- * (Evaluate = createSubproblem + SDPbound, src/evaluate.c:11-20).
- * Per node b: xfixed/solX rows of length n0-1 (src/biqbin_cpp_api.h:54-63),
- * LB[b] the incumbent value at batch start, diff[b] the root gap used by the
- * give-up test, is_root[b] the reference's rank == 0 semantics, seeds[b] the
- * node's random stream.  Outputs per node: bound (f + fixedvalue), fracsol
- * (n0-1), xbest (n0-1, best heuristic cut; valid if improved), LB_out
- * (LB raised by the node's own heuristic), diff_out, status, and 8 stats
- * ints (prune, giveup, outer iterations, IPM calls, IPM iterations, max cuts,
- * max bundle size, n).  Any output pointer may be NULL. */
+/* Bounds `nodes` (<= max_nodes) B&B nodes with the algorithm of
+ * src/bounding.c:24-342 (Evaluate = createSubproblem + SDPbound,
+ * src/evaluate.c:11-20).  Per node b: xfixed/solX rows of length n0-1
+ * (src/biqbin_cpp_api.h:54-63), LB[b] the incumbent value at its start,
+ * diff[b] the root gap used by the give-up test, is_root[b] the reference's
+ * rank == 0 semantics, seeds[b] the node's random stream.  Outputs per node:
+ * bound (f + fixedvalue), fracsol (n0-1), xbest (n0-1, best heuristic cut;
+ * valid if improved), LB_out (LB raised by the node's own heuristic),
+ * diff_out, status, and 8 stats ints (prune, giveup, outer iterations, IPM
+ * calls, IPM iterations, max cuts, max bundle size, n).  Any output pointer
+ * may be NULL.  A node's results depend only on its own inputs. */
 int bqb_bound_batch(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
                     const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
                     double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
                     int *status, int *stats);
+/* Same for any number of nodes: they stream through the max_nodes slots,
+ * a finished node's slot is refilled after every outer round. */
+int bqb_bound_stream(bqb_ctx *c, int count, const int *xfixed, const int *solX, const double *LB,
+                     const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
+                     double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
+                     int *status, int *stats);
 
-/* Stepwise use of a context (tests and diagnostics): load a batch, run one
- * phase (0-11 control phases; 100 basic IPM, 101 IPM on L0, 102 heuristic,
+/* Incremental use (the node queue): push a node into a free slot (returns
+ * the slot, -1 if none is free; tag >= 0 identifies it), advance every slot
+ * by one outer round (nodes pushed since the last call first run their
+ * basic bound), pop a finished node (returns 1, or 0 if none is waiting;
+ * outputs as bqb_bound_batch for one node). */
+int bqb_ctx_push(bqb_ctx *c, long long tag, const int *xfixed, const int *solX, double LB,
+                 double diff, int is_root, uint64_t seed);
+int bqb_ctx_advance(bqb_ctx *c);
+int bqb_ctx_pop(bqb_ctx *c, long long *tag, double *bound, double *fracsol, int *xbest, double *LB_out,
+                int *improved, double *diff_out, int *status, int *stats);
+int bqb_ctx_busy(const bqb_ctx *c);       /* occupied slots */
+int bqb_ctx_capacity(const bqb_ctx *c);   /* max_nodes */
+
+/* Stepwise use of a context (tests and diagnostics): load a batch into
+ * slots 0..nodes-1, run one phase (0-14 control phases; 100 basic IPM,
+ * 101 IPM on L0, 102 heuristic,
  * 103 triangle update, 104/105 pentagonal/heptagonal update, 106 op_Bt
  * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP,
  * 109 bundle step), run the whole bound, read results, and copy one node's

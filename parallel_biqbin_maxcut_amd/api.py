@@ -128,7 +128,7 @@ class NodeState(C.Structure):
         "n", "status", "active", "prune", "giveup", "done", "count", "bdl_iter", "NIneq",
         "NPentIneq", "NHeptaIneq", "bundle", "sep_tri", "sep_pent", "sep_hept", "heur_idx",
         "improved", "ipm_calls", "ipm_iters", "max_cuts", "max_bundle_seen", "is_root", "serious",
-        "cuts_changed", "off_pent", "off_hept", "pad0")] + [(f, C.c_double) for f in (
+        "cuts_changed", "off_pent", "off_hept", "stage")] + [(f, C.c_double) for f in (
         "fixedvalue", "f", "f_test", "t", "bound", "basic_bound", "oldf", "LB", "diff", "viol3",
         "viol5", "gap", "del_", "lmax")] + [("seed", C.c_uint64)]
 
@@ -138,7 +138,8 @@ BUF = dict(state=0, L=1, L0=2, X=3, Xt=4, tri=5, tri_y=6, pent=7, pent_y=8, hept
            kg_lval=20, fracsol=21, xbest=22, xheur=23, mask=24, phi=25, Xb=26, nvec=27, dgamma=28,
            sa_val=29)
 PH = dict(SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OUTER_A=5, OUTER_B=6,
-          SEP_PENT=7, SEP_HEPT=8, OUTER_C=9, REBUILD=10, FINISH=11, IPM_BASIC=100, IPM_L0=101,
+          SEP_PENT=7, SEP_HEPT=8, OUTER_C=9, REBUILD=10, FINISH=11, FCT_FINISH=12, LOAD=13,
+          BUNDLE_NEXT=14, IPM_BASIC=100, IPM_L0=101,
           HEUR=102, TRISEP=103, PENT=104, HEPT=105, CSR=106, OPBT=107, QP=108, STEP=109)
 
 
@@ -259,6 +260,13 @@ def _declare_ctx(L):
                                   N.iptr, N.dptr, N.iptr, N.dptr, N.iptr, N.iptr]
     L.bqb_fct_eval.argtypes = [vp, C.c_int, N.dptr, C.c_int, N.iptr, C.c_int, N.iptr, C.c_int, N.iptr, N.dptr,
                                N.dptr, N.dptr, N.dptr, N.iptr]
+    L.bqb_bound_stream.argtypes = L.bqb_bound_batch.argtypes
+    L.bqb_ctx_push.argtypes = [vp, C.c_longlong, N.iptr, N.iptr, C.c_double, C.c_double, C.c_int, C.c_uint64]
+    L.bqb_ctx_advance.argtypes = [vp]
+    L.bqb_ctx_pop.argtypes = [vp, C.POINTER(C.c_longlong), N.dptr, N.dptr, N.iptr, N.dptr, N.iptr, N.dptr,
+                              N.iptr, N.iptr]
+    L.bqb_ctx_busy.argtypes = [vp]
+    L.bqb_ctx_capacity.argtypes = [vp]
     L.bqb_ctx_load.argtypes = [vp, C.c_int, N.iptr, N.iptr, N.dptr, N.dptr, N.iptr, u64p]
     L.bqb_ctx_run.argtypes = [vp, C.c_int, C.c_int]
     L.bqb_ctx_bound.argtypes = [vp]
@@ -307,14 +315,17 @@ class BoundContext:
         sd = np.ascontiguousarray(np.broadcast_to(seeds, (nodes,)), dtype=np.uint64)
         return xf, sx, lb, df, rt, sd
 
-    def bound(self, xfixed, solX, LB=0.0, diff=0.0, is_root=0, seeds=0):
+    def bound(self, xfixed, solX, LB=0.0, diff=0.0, is_root=0, seeds=0, stream=False):
+        """Bound the nodes (rows of xfixed/solX): bqb_bound_batch, or
+        bqb_bound_stream (any count, slots refilled) if stream."""
         nodes = int(np.shape(xfixed)[0])
         xf, sx, lb, df, rt, sd = self._arrs(nodes, self.n0, xfixed, solX, LB, diff, is_root, seeds)
         B = self.n0 - 1
         out = dict(bound=np.zeros(nodes), fracsol=np.zeros((nodes, B)), xbest=np.zeros((nodes, B), np.int32),
                    LB=np.zeros(nodes), improved=np.zeros(nodes, np.int32), diff=np.zeros(nodes),
                    status=np.zeros(nodes, np.int32), stats=np.zeros((nodes, 8), np.int32))
-        N.check(N.lib().bqb_bound_batch(
+        fn = N.lib().bqb_bound_stream if stream else N.lib().bqb_bound_batch
+        N.check(fn(
             self.h, nodes, _i(xf), _i(sx), _d(lb), _d(df), _i(rt), sd.ctypes.data_as(C.POINTER(C.c_uint64)),
             _d(out["bound"]), _d(out["fracsol"]), _i(out["xbest"]), _d(out["LB"]), _i(out["improved"]),
             _d(out["diff"]), _i(out["status"]), _i(out["stats"])))

@@ -15,12 +15,13 @@
  *   src/bab_functions.c:387-421  getBranchingVariable
  *   src/heap.c                   max-heap on upper_bound (ties: -1)
  *
- * The MPI master/worker loop becomes one queue on one GPU: the best `batch`
- * nodes are popped together (nodes whose bound no longer beats the
- * incumbent are dropped), bounded in one bqb_bound_batch call against the
- * incumbent at that moment, the incumbent is raised from their heuristic
- * solutions in node order, and each survivor is branched.  Children carry
- * the parent's bound until they are evaluated.  With batch = 1 this is the
+ * The MPI master/worker loop becomes one queue on one GPU: a context with
+ * `batch` node slots is kept full with the best open nodes (nodes whose
+ * bound no longer beats the incumbent are dropped), every slot advances one
+ * outer round of SDPbound per step, and a node that has finished raises the
+ * incumbent with its heuristic solution and is branched at once.  Each node
+ * reads the incumbent of the moment it entered its slot.  Children carry the
+ * parent's bound until they are evaluated.  With batch = 1 this is the
  * sequential B&B of the reference with one worker.
  */
 #include <math.h>
@@ -270,45 +271,40 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
     double LB = 0.0;                       /* bab_functions.c:26-35 */
     memset(sol, 0, sizeof(int) * B);
     double diff = 0.0;
-    qnode **cur = calloc(batch, sizeof(qnode *));
-    int *xf = malloc(sizeof(int) * (size_t)batch * B);
-    int *sx = malloc(sizeof(int) * (size_t)batch * B);
-    double *lb = malloc(sizeof(double) * batch), *df = malloc(sizeof(double) * batch);
-    int *rt = malloc(sizeof(int) * batch);
-    uint64_t *sd = malloc(sizeof(uint64_t) * batch);
-    double *bound = malloc(sizeof(double) * batch), *frac = malloc(sizeof(double) * (size_t)batch * B);
-    int *xb = malloc(sizeof(int) * (size_t)batch * B);
-    double *lbo = malloc(sizeof(double) * batch), *dfo = malloc(sizeof(double) * batch);
-    int *imp = malloc(sizeof(int) * batch), *st = malloc(sizeof(int) * batch);
-    if (!cur || !xf || !sx || !lb || !df || !rt || !sd || !bound || !frac || !xb || !lbo || !dfo || !imp || !st) {
-        bqb_set_error("calloc");
-        goto done;
-    }
+    qnode **inflight = calloc(batch, sizeof(qnode *));      /* by tag */
+    int *tag_free = malloc(sizeof(int) * batch), ntag = batch;
+    double *frac = malloc(sizeof(double) * B);
+    int *xb = malloc(sizeof(int) * B);
+    if (!inflight || !tag_free || !frac || !xb) { bqb_set_error("calloc"); goto done; }
+    for (int i = 0; i < batch; ++i) tag_free[i] = batch - 1 - i;
 
     /* root node (bab_functions.c:39-69) */
     {
         qnode *root = node_new(NULL, B, next_id++);
         if (!root) goto done;
-        memcpy(xf, root->xfixed, sizeof(int) * B);
-        memcpy(sx, root->solX, sizeof(int) * B);
-        lb[0] = LB; df[0] = 0.0; rt[0] = 1; sd[0] = bqb_key(seed, root->id);
-        if (bqb_bound_batch(ctx, 1, xf, sx, lb, df, rt, sd, bound, frac, xb, lbo, imp, dfo, st, NULL)) {
+        double bound, lbo, dfo;
+        int imp, st, stats[8];
+        const double LB0 = LB, df0 = 0.0;
+        const int is_root = 1;
+        const uint64_t sd = bqb_key(seed, root->id);
+        if (bqb_bound_batch(ctx, 1, root->xfixed, root->solX, &LB0, &df0, &is_root, &sd, &bound, frac, xb,
+                            &lbo, &imp, &dfo, &st, stats)) {
             free(root);
             goto done;
         }
         out->bound_ms += bqb_ctx_last_ms(ctx);
         out->nodes = 1;
         out->batches = 1;
-        if (st[0]) {
-            out->status = st[0];
-            bqb_set_error("root node bound failed with status %d", st[0]);
+        if (st) {
+            out->status = st;
+            bqb_set_error("root node bound failed with status %d", st);
             free(root);
             goto done;
         }
-        if (imp[0] && lbo[0] > LB) { LB = lbo[0]; memcpy(sol, xb, sizeof(int) * B); }
-        diff = dfo[0];
-        root->upper_bound = bound[0];
-        out->root_bound = bound[0];
+        if (imp && lbo > LB) { LB = lbo; memcpy(sol, xb, sizeof(int) * B); }
+        diff = dfo;
+        root->upper_bound = bound;
+        out->root_bound = bound;
         memcpy(root->fracsol, frac, sizeof(double) * B);
         if (LB + 1.0 < root->upper_bound && !params->root) {
             const int ic = branching_variable(root, B, params->branchingStrategy);
@@ -322,41 +318,43 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
         free(root);
     }
 
-    while (h.used > 0) {
+    /* node queue: refill free slots with the best open nodes after every
+     * outer round, branch every node as soon as its bound is known */
+    for (;;) {
         if (params->time_limit > 0 && now_s() - t0 > params->time_limit) { out->stopped = 1; break; }
         if (h.used > out->max_queue) out->max_queue = (int)h.used;
-        int nb = 0;
-        while (nb < batch && h.used > 0) {
+        while (ntag > 0 && h.used > 0) {
             qnode *x = heap_pop(&h);
-            if (LB + 1.0 < x->upper_bound) cur[nb++] = x;
-            else free(x);                 /* cannot beat the incumbent any more */
-        }
-        if (!nb) break;
-        for (int b = 0; b < nb; ++b) {
-            memcpy(xf + (size_t)b * B, cur[b]->xfixed, sizeof(int) * B);
-            memcpy(sx + (size_t)b * B, cur[b]->solX, sizeof(int) * B);
-            lb[b] = LB; df[b] = diff; rt[b] = 0; sd[b] = bqb_key(seed, cur[b]->id);
-        }
-        if (bqb_bound_batch(ctx, nb, xf, sx, lb, df, rt, sd, bound, frac, xb, lbo, imp, dfo, st, NULL)) {
-            for (int b = 0; b < nb; ++b) free(cur[b]);
-            goto done;
-        }
-        out->bound_ms += bqb_ctx_last_ms(ctx);
-        out->nodes += nb;
-        out->batches += 1;
-        for (int b = 0; b < nb; ++b) {
-            if (st[b] && !out->status) {
-                out->status = st[b];
-                bqb_set_error("node bound failed with status %d", st[b]);
+            if (!(LB + 1.0 < x->upper_bound)) { free(x); continue; }   /* cannot beat the incumbent */
+            const int t = tag_free[--ntag];
+            if (bqb_ctx_push(ctx, t, x->xfixed, x->solX, LB, diff, 0, bqb_key(seed, x->id)) < 0) {
+                free(x);
+                goto done;
             }
-            if (imp[b] && lbo[b] > LB) { LB = lbo[b]; memcpy(sol, xb + (size_t)b * B, sizeof(int) * B); }
-            cur[b]->upper_bound = bound[b];
-            memcpy(cur[b]->fracsol, frac + (size_t)b * B, sizeof(double) * B);
+            inflight[t] = x;
         }
-        int fail = out->status != 0;
-        for (int b = 0; b < nb; ++b) {
-            qnode *x = cur[b];
-            if (!fail && LB + 1.0 < x->upper_bound) {
+        if (bqb_ctx_busy(ctx) == 0) break;
+        const double ta = now_s();
+        if (bqb_ctx_advance(ctx)) goto done;
+        out->bound_ms += 1e3 * (now_s() - ta);
+        out->batches += 1;
+        long long t;
+        double bound, lbo, dfo;
+        int imp, st;
+        int fail = 0;
+        while (bqb_ctx_pop(ctx, &t, &bound, frac, xb, &lbo, &imp, &dfo, &st, NULL)) {
+            qnode *x = inflight[t];
+            inflight[t] = NULL;
+            tag_free[ntag++] = (int)t;
+            out->nodes += 1;
+            if (st && !out->status) {
+                out->status = st;
+                bqb_set_error("node bound failed with status %d", st);
+            }
+            if (imp && lbo > LB) { LB = lbo; memcpy(sol, xb, sizeof(int) * B); }
+            x->upper_bound = bound;
+            memcpy(x->fracsol, frac, sizeof(double) * B);
+            if (!out->status && !fail && LB + 1.0 < x->upper_bound) {
                 const int ic = branching_variable(x, B, params->branchingStrategy);
                 for (int v = 0; v <= 1 && ic >= 0; ++v) {
                     qnode *c = node_new(x, B, next_id++);
@@ -367,14 +365,15 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
             }
             free(x);
         }
-        if (fail) goto done;
+        if (fail || out->status) goto done;
     }
     rc = 0;
 done:
     while (h.used > 0) free(heap_pop(&h));
     free(h.d);
-    free(cur); free(xf); free(sx); free(lb); free(df); free(rt); free(sd); free(bound);
-    free(frac); free(xb); free(lbo); free(dfo); free(imp); free(st);
+    if (inflight)
+        for (int i = 0; i < batch; ++i) free(inflight[i]);
+    free(inflight); free(tag_free); free(frac); free(xb);
     bqb_ctx_destroy(ctx);
     out->best = LB;
     out->seconds = now_s() - t0;

@@ -40,11 +40,16 @@ typedef struct NodeState {
     int cuts_changed;   /* op_Bt gather lists must be rebuilt */
     int off_pent, off_hept;   /* gamma offsets of the pentagonal / heptagonal
                                  multipliers before this round's purge */
-    int pad0;
+    int stage;          /* slot life cycle: BQB_STAGE_* */
     double fixedvalue, f, f_test, t, bound, basic_bound, oldf;
     double LB, diff, viol3, viol5, gap, del, lmax;
     uint64_t seed;
 } NodeState;
+
+/* Slot life cycle: a node is loaded (NEW), runs the basic bound and the
+ * first separation, enters the outer loop (LOOP) and leaves it when it is
+ * pruned, given up or finished (active = 0); the host then frees the slot. */
+enum { BQB_STAGE_FREE = 0, BQB_STAGE_NEW = 1, BQB_STAGE_LOOP = 2 };
 
 /* Cut records (src/biqbin.h:81-107 hold the same information). */
 typedef struct TriCut { int i, j, k, type; } TriCut;
@@ -95,6 +100,10 @@ typedef struct BatchPtrs {
     double *sa_val;                 /* [nodes*chains] */
     int *sa_perm;                   /* [nodes*chains*7] */
     int *sa_status;                 /* [nodes*chains] */
+    /* staging of newly loaded nodes (compact, [nload]) and the slot map */
+    NodeState *ld_st;               /* [nodes] */
+    int *ld_xf, *ld_sx;             /* [nodes*(n0-1)] */
+    int *ld_of_slot;                /* [nodes]: index into the staging, or -1 */
 } BatchPtrs;
 
 #define BQB_SA_CHAINS 256           /* >= max(Pent_Trials*3, Hepta_Trials*4) supported */

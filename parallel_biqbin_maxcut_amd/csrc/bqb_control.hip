@@ -96,10 +96,25 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
     const int B = d.n0 - 1;
     const size_t slab = (size_t)d.ld * d.ld;
 
+    if (phase == PH_LOAD) {
+        // staged node -> slot; every slot's mask says whether it was loaded
+        const int k = p.ld_of_slot[node];
+        if (k >= 0) {
+            if (threadIdx.x == 0) *sp = p.ld_st[k];
+            for (int i = threadIdx.x; i < B; i += CT) {
+                const_cast<int *>(p.xfixed)[(size_t)node * B + i] = p.ld_xf[(size_t)k * B + i];
+                const_cast<int *>(p.solX)[(size_t)node * B + i] = p.ld_sx[(size_t)k * B + i];
+            }
+        }
+        if (threadIdx.x == 0) p.mask[node] = (k >= 0) ? 1 : 0;
+        return;
+    }
+
     if (phase == PH_SETUP) {
         // createSubproblem (evaluate.c:38-115) into the zero-padded slab,
         // getFixedValue (evaluate.c:123-138).  Integer-valued data: every
         // sum below is exact, so the parallel order does not matter.
+        if (sp->stage != BQB_STAGE_NEW || !p.mask[node]) return;
         const int *xf = p.xfixed + (size_t)node * B;
         const int *sx = p.solX + (size_t)node * B;
         double *L = p.L + node * slab;
@@ -176,6 +191,27 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
     NodeState s = *sp;   // snapshot (written back by thread 0 where changed)
     __syncthreads();
 
+    // Each phase acts on the nodes of one stage; every other slot has its
+    // mask cleared so the kernels that follow skip it.
+    {
+        int want = -1;
+        switch (phase) {
+        case PH_POST_BASIC: case PH_CHECK_BASIC: case PH_BUNDLE_INIT: want = BQB_STAGE_NEW; break;
+        case PH_OUTER_BEGIN: case PH_OUTER_A: case PH_OUTER_B: case PH_SEP_PENT: case PH_SEP_HEPT:
+        case PH_OUTER_C: case PH_REBUILD: case PH_BUNDLE_NEXT: want = BQB_STAGE_LOOP; break;
+        default: break;
+        }
+        if (want >= 0 && s.stage != want) {
+            if (threadIdx.x == 0) p.mask[node] = 0;
+            return;
+        }
+    }
+
+    if (phase == PH_BUNDLE_NEXT) {
+        if (threadIdx.x == 0) p.mask[node] = (s.active && arg < s.bdl_iter) ? 1 : 0;
+        return;
+    }
+
     if (phase == PH_POST_BASIC) {
         // bounding.c:73-103: f from the basic IPM, fracsol, heuristic mask
         if (!s.active) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
@@ -213,7 +249,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
 
     if (phase == PH_BUNDLE_INIT) {
         // bounding.c:127-163
-        if (!s.active) return;
+        if (!s.active) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
         const int m = s.NIneq;
         double *gam = p.gamma + (size_t)node * d.max_m;
         double *g = p.g + (size_t)node * d.max_m;
@@ -232,14 +268,24 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             sp->max_bundle_seen = 1;
             sp->t = 0.5 * (s.bound - s.LB) / (m * s.viol3 * s.viol3);
             sp->cuts_changed = 1;
+            sp->stage = BQB_STAGE_LOOP;
             p.mask[node] = 1;
         }
         return;
     }
 
     if (phase == PH_OUTER_BEGIN) {
+        // bounding.c:170-174; the bundle length of outer round c is
+        // init_bundle_iter + floor(c/2), at most max_bundle_iter
+        // (bounding.c:323-325 add c % 2 after every round)
         if (threadIdx.x == 0) {
-            if (s.active) { sp->count = s.count + 1; sp->oldf = s.f; sp->bdl_iter = arg; }
+            if (s.active) {
+                const int c = s.count + 1;
+                const int bdl = P.init_bundle_iter + c / 2;
+                sp->count = c;
+                sp->oldf = s.f;
+                sp->bdl_iter = bdl < P.max_bundle_iter ? bdl : P.max_bundle_iter;
+            }
             p.mask[node] = s.active;
         }
         return;
@@ -338,7 +384,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
     if (phase == PH_REBUILD) {
         // bounding.c:278-321: extend the bundle with fct_eval(gamma) after a
         // cut change; G is recomputed for the new cut set.
-        if (!s.active) return;
+        if (!s.active) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
         const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
         if (s.bundle >= d.max_bundle) {
             if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
@@ -391,7 +437,8 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
 
     if (phase == PH_FINISH) {
         if (threadIdx.x == 0) {
-            if (sp->active) { sp->active = 0; sp->bound = s.f + s.fixedvalue; }
+            if (s.active && s.stage == BQB_STAGE_LOOP) { sp->active = 0; sp->bound = s.f + s.fixedvalue; }
+            p.mask[node] = 0;
         }
         return;
     }

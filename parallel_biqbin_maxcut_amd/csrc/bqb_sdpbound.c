@@ -53,9 +53,54 @@ struct bqb_ctx {
     hipEvent_t *pev;
     int pev_cap, pev_n;
     unsigned long long *d_flops;
+    /* slot engine: occupied slots carry a caller tag (-1 = free); nodes
+     * pushed since the last advance wait in the host staging arrays */
+    long long *tag;            /* [max_nodes] */
+    int *free_stack, nfree;    /* free slots, lowest on top */
+    int *of_slot;              /* [max_nodes] staging index or -1 */
+    NodeState *ld_st;          /* [max_nodes] staged states */
+    int *ld_xf, *ld_sx;        /* [max_nodes*(n0-1)] staged rows */
+    int nload;
+    int loaded;                /* nodes loaded by bqb_ctx_load still need their basic bound */
+    int hi;                    /* 1 + highest occupied slot */
+    double *h_frac;            /* [max_nodes*(n0-1)] */
+    int *h_xb;                 /* [max_nodes*(n0-1)] */
+    /* finished nodes not yet popped (FIFO) */
+    struct done_node *done;
+    int ndone, done_head, done_cap;
+};
+
+struct done_node {
+    long long tag;
+    NodeState st;
+    double *frac;    /* n0-1 */
+    int *xb;         /* n0-1 */
 };
 
 #define CK(call, what) do { if (bqb_check((call), (what))) goto fail; } while (0)
+
+static void free_host(bqb_ctx *c)
+{
+    free(c->h_st); free(c->tag); free(c->free_stack); free(c->of_slot); free(c->ld_st);
+    free(c->ld_xf); free(c->ld_sx); free(c->h_frac); free(c->h_xb);
+    for (int i = 0; i < c->done_cap; ++i) { free(c->done[i].frac); free(c->done[i].xb); }
+    free(c->done);
+}
+
+/* All slots free, nothing staged, no finished nodes pending. */
+static void engine_reset(bqb_ctx *c)
+{
+    for (int i = 0; i < c->max_nodes; ++i) {
+        c->tag[i] = -1;
+        c->of_slot[i] = -1;
+        c->free_stack[i] = c->max_nodes - 1 - i;
+    }
+    c->nfree = c->max_nodes;
+    c->nload = 0;
+    c->loaded = 0;
+    c->hi = 0;
+    c->ndone = c->done_head = 0;
+}
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -152,6 +197,10 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         {(void **)&c->p.sa_perm, sizeof(int) * N * BQB_SA_CHAINS * 7},
         {(void **)&c->p.sa_status, sizeof(int) * N * BQB_SA_CHAINS},
         {(void **)&c->d_flops, sizeof(unsigned long long)},
+        {(void **)&c->p.ld_st, sizeof(NodeState) * N},
+        {(void **)&c->p.ld_xf, sizeof(int) * N * B},
+        {(void **)&c->p.ld_sx, sizeof(int) * N * B},
+        {(void **)&c->p.ld_of_slot, sizeof(int) * N},
     };
     const int nitems = (int)(sizeof items / sizeof items[0]);
     size_t total = 0;
@@ -172,11 +221,24 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     CK(hipEventCreate(&c->ev0), "hipEventCreate");
     CK(hipEventCreate(&c->ev1), "hipEventCreate");
     c->h_st = calloc(N, sizeof(NodeState));
-    if (!c->h_st) { bqb_set_error("calloc"); goto fail; }
+    c->tag = malloc(sizeof(long long) * N);
+    c->free_stack = malloc(sizeof(int) * N);
+    c->of_slot = malloc(sizeof(int) * N);
+    c->ld_st = calloc(N, sizeof(NodeState));
+    c->ld_xf = malloc(sizeof(int) * N * B);
+    c->ld_sx = malloc(sizeof(int) * N * B);
+    c->h_frac = malloc(sizeof(double) * N * B);
+    c->h_xb = malloc(sizeof(int) * N * B);
+    if (!c->h_st || !c->tag || !c->free_stack || !c->of_slot || !c->ld_st || !c->ld_xf || !c->ld_sx ||
+        !c->h_frac || !c->h_xb) {
+        bqb_set_error("calloc");
+        goto fail;
+    }
+    engine_reset(c);
     return c;
 fail:
     if (c->arena) hipFree(c->arena);
-    free(c->h_st);
+    free_host(c);
     free(c);
     return NULL;
 }
@@ -194,11 +256,11 @@ void bqb_ctx_destroy(bqb_ctx *c)
     if (!c) return;
     hipStreamSynchronize(c->stream);
     free_events(c);
+    free_host(c);
     hipFree(c->arena);
     hipStreamDestroy(c->stream);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
-    free(c->h_st);
     free(c);
 }
 
@@ -251,63 +313,169 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     case 108: return bqb_check(bqb_launch_bundle_qp(d, p, s), "qp launch");
     case 109: return bqb_check(bqb_launch_bundle_step(d, p, s), "step launch");
     default:
-        if (what >= 0 && what <= PH_FCT_FINISH)
+        if (what >= 0 && what <= PH_BUNDLE_NEXT)
             return bqb_check(bqb_launch_control(d, p, what, arg, s), "control launch");
         bqb_set_error("unknown phase %d", what);
         return -1;
     }
 }
 
-static int count_active(bqb_ctx *c)
-{
-    if (bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * c->d.nodes,
-                                 hipMemcpyDeviceToHost, c->stream), "D2H state"))
-        return -1;
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
-    int a = 0;
-    for (int i = 0; i < c->d.nodes; ++i) a += c->h_st[i].active ? 1 : 0;
-    return a;
-}
-
 #define RUN(what, arg) do { if (bqb_ctx_run(c, (what), (arg))) return -1; } while (0)
 
-/* Upload a batch: fixings and the per-node inputs of the bound. */
-int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
-                 const double *diff, const int *is_root, const uint64_t *seeds)
+/* ---- slot engine ---------------------------------------------------------
+ * Nodes are pushed into free slots, advanced one outer round at a time and
+ * popped when they have finished.  bqb_ctx_advance() runs, in order:
+ *   - for the nodes pushed since the last call: load, createSubproblem,
+ *     basic IPM, heuristic, prune / give-up test, first triangle
+ *     separation, bundle start (bounding.c:24-163);
+ *   - one outer round (bounding.c:170-326) for every node in the loop,
+ *     including the ones just started: each node runs its own number of
+ *     bundle iterations;
+ * and then moves the nodes that have left SDPbound to the finished list.
+ * A node's bound depends only on its own inputs, so it does not matter which
+ * slot it occupies or which nodes share the launches with it. */
+
+int bqb_ctx_push(bqb_ctx *c, long long tag, const int *xfixed, const int *solX, double LB,
+                 double diff, int is_root, uint64_t seed)
 {
-    if (nodes < 1 || nodes > c->max_nodes) { bqb_set_error("batch of %d nodes (capacity %d)", nodes, c->max_nodes); return -1; }
-    c->d.nodes = nodes;
+    if (tag < 0) { bqb_set_error("push: tag must be >= 0"); return -1; }
+    if (c->nfree == 0) { bqb_set_error("push: all %d slots are busy", c->max_nodes); return -1; }
+    const int slot = c->free_stack[--c->nfree];
+    const int k = c->nload++;
     const size_t B = (size_t)(c->d.n0 - 1);
-    memset(c->h_st, 0, sizeof(NodeState) * nodes);
-    for (int i = 0; i < nodes; ++i) {
-        c->h_st[i].LB = LB ? LB[i] : 0.0;
-        c->h_st[i].diff = diff ? diff[i] : 0.0;
-        c->h_st[i].is_root = is_root ? is_root[i] : 0;
-        c->h_st[i].seed = seeds ? seeds[i] : (uint64_t)i;
-    }
-    if (bqb_check(hipMemcpyAsync((void *)c->p.xfixed, xfixed, sizeof(int) * nodes * B, hipMemcpyHostToDevice, c->stream), "H2D xfixed")) return -1;
-    if (bqb_check(hipMemcpyAsync((void *)c->p.solX, solX, sizeof(int) * nodes * B, hipMemcpyHostToDevice, c->stream), "H2D solX")) return -1;
-    if (bqb_check(hipMemcpyAsync(c->p.st, c->h_st, sizeof(NodeState) * nodes, hipMemcpyHostToDevice, c->stream), "H2D state")) return -1;
+    NodeState *st = &c->ld_st[k];
+    memset(st, 0, sizeof *st);
+    st->LB = LB;
+    st->diff = diff;
+    st->is_root = is_root;
+    st->seed = seed;
+    st->stage = BQB_STAGE_NEW;
+    memcpy(c->ld_xf + k * B, xfixed, sizeof(int) * B);
+    memcpy(c->ld_sx + k * B, solX, sizeof(int) * B);
+    c->of_slot[slot] = k;
+    c->tag[slot] = tag;
+    if (slot + 1 > c->hi) c->hi = slot + 1;
+    return slot;
+}
+
+int bqb_ctx_busy(const bqb_ctx *c) { return c->max_nodes - c->nfree; }
+int bqb_ctx_capacity(const bqb_ctx *c) { return c->max_nodes; }
+
+/* Copy the staged nodes into their slots (PH_LOAD) and set d.nodes. */
+static int upload_staged(bqb_ctx *c)
+{
+    const size_t B = (size_t)(c->d.n0 - 1);
+    hipStream_t s = c->stream;
+    c->d.nodes = c->hi;
+    if (bqb_check(hipMemcpyAsync(c->p.ld_of_slot, c->of_slot, sizeof(int) * c->hi, hipMemcpyHostToDevice, s), "H2D slots")) return -1;
+    if (bqb_check(hipMemcpyAsync(c->p.ld_st, c->ld_st, sizeof(NodeState) * c->nload, hipMemcpyHostToDevice, s), "H2D staged state")) return -1;
+    if (bqb_check(hipMemcpyAsync(c->p.ld_xf, c->ld_xf, sizeof(int) * B * c->nload, hipMemcpyHostToDevice, s), "H2D staged xfixed")) return -1;
+    if (bqb_check(hipMemcpyAsync(c->p.ld_sx, c->ld_sx, sizeof(int) * B * c->nload, hipMemcpyHostToDevice, s), "H2D staged solX")) return -1;
+    if (bqb_ctx_run(c, PH_LOAD, 0)) return -1;
+    for (int i = 0; i < c->hi; ++i) c->of_slot[i] = -1;
+    c->nload = 0;
+    /* the host copies of the staging must stay valid until the copies ran */
+    return bqb_check(hipStreamSynchronize(s), "sync");
+}
+
+static int fetch_states(bqb_ctx *c)
+{
+    if (bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * c->hi, hipMemcpyDeviceToHost, c->stream), "D2H state"))
+        return -1;
     return bqb_check(hipStreamSynchronize(c->stream), "sync");
 }
 
-/* The phase sequence of bounding.c:24-342 for the loaded batch. */
-int bqb_ctx_bound(bqb_ctx *c)
+static int done_push(bqb_ctx *c, long long tag, const NodeState *st, const double *frac, const int *xb)
+{
+    const int B = c->d.n0 - 1;
+    if (c->ndone == c->done_cap) {
+        const int cap = c->done_cap ? 2 * c->done_cap : c->max_nodes;
+        struct done_node *d = malloc(sizeof(struct done_node) * cap);
+        if (!d) { bqb_set_error("calloc"); return -1; }
+        for (int i = 0; i < c->ndone; ++i) d[i] = c->done[(c->done_head + i) % c->done_cap];
+        for (int i = c->ndone; i < cap; ++i) {
+            d[i].frac = malloc(sizeof(double) * B);
+            d[i].xb = malloc(sizeof(int) * B);
+            if (!d[i].frac || !d[i].xb) {
+                for (int j = c->ndone; j <= i; ++j) { free(d[j].frac); free(d[j].xb); }
+                free(d);
+                bqb_set_error("calloc");
+                return -1;
+            }
+        }
+        free(c->done);
+        c->done = d;
+        c->done_cap = cap;
+        c->done_head = 0;
+    }
+    struct done_node *e = &c->done[(c->done_head + c->ndone) % c->done_cap];
+    e->tag = tag;
+    e->st = *st;
+    memcpy(e->frac, frac, sizeof(double) * B);
+    memcpy(e->xb, xb, sizeof(int) * B);
+    ++c->ndone;
+    return 0;
+}
+
+/* Move the nodes that have left SDPbound to the finished list, free slots. */
+static int collect(bqb_ctx *c)
+{
+    const size_t B = (size_t)(c->d.n0 - 1);
+    int any = 0;
+    for (int i = 0; i < c->hi; ++i)
+        if (c->tag[i] >= 0 && !c->h_st[i].active) { any = 1; break; }
+    if (!any) return 0;
+    if (bqb_check(hipMemcpyAsync(c->h_frac, c->p.fracsol, sizeof(double) * B * c->hi, hipMemcpyDeviceToHost, c->stream), "D2H fracsol")) return -1;
+    if (bqb_check(hipMemcpyAsync(c->h_xb, c->p.xbest, sizeof(int) * B * c->hi, hipMemcpyDeviceToHost, c->stream), "D2H xbest")) return -1;
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    for (int i = 0; i < c->hi; ++i) {
+        if (c->tag[i] < 0 || c->h_st[i].active) continue;
+        if (done_push(c, c->tag[i], &c->h_st[i], c->h_frac + i * B, c->h_xb + i * B)) return -1;
+        c->tag[i] = -1;
+    }
+    /* free slots, lowest index on top of the stack */
+    c->nfree = 0;
+    for (int i = c->max_nodes - 1; i >= 0; --i)
+        if (c->tag[i] < 0) c->free_stack[c->nfree++] = i;
+    c->hi = 0;
+    for (int i = 0; i < c->max_nodes; ++i)
+        if (c->tag[i] >= 0) c->hi = i + 1;
+    return 0;
+}
+
+int bqb_ctx_advance(bqb_ctx *c)
 {
     const BqbParams *P = &c->params;
-    RUN(PH_SETUP, 0);
-    RUN(100, 0);                 /* basic IPM: L -> X */
-    RUN(PH_POST_BASIC, 0);
-    RUN(102, 0);                 /* heuristic */
-    RUN(PH_CHECK_BASIC, 0);
-    RUN(103, 0);                 /* first triangle separation */
-    RUN(PH_BUNDLE_INIT, 0);
-    RUN(106, 0);                 /* op_Bt gather lists */
-    int bdl = P->init_bundle_iter;
-    const int max_count = P->max_outer_iter + P->extra_iter;
-    for (int count = 1; count <= max_count; ++count) {
-        RUN(PH_OUTER_BEGIN, bdl);
-        for (int b = 0; b < bdl; ++b) {
+    if (c->nload) {
+        if (upload_staged(c)) return -1;
+        c->loaded = 1;
+    }
+    if (c->hi == 0) return 0;
+    c->d.nodes = c->hi;
+    if (c->loaded) {
+        RUN(PH_SETUP, 0);
+        RUN(100, 0);             /* basic IPM: L -> X */
+        RUN(PH_POST_BASIC, 0);
+        RUN(102, 0);             /* heuristic */
+        RUN(PH_CHECK_BASIC, 0);
+        RUN(103, 0);             /* first triangle separation */
+        RUN(PH_BUNDLE_INIT, 0);
+        RUN(106, 0);             /* op_Bt gather lists */
+        c->loaded = 0;
+    }
+    if (fetch_states(c)) return -1;
+    int bdl_max = 0;
+    for (int i = 0; i < c->hi; ++i) {
+        const NodeState *s = &c->h_st[i];
+        if (c->tag[i] < 0 || !s->active || s->stage != BQB_STAGE_LOOP) continue;
+        int bdl = P->init_bundle_iter + (s->count + 1) / 2;
+        if (bdl > P->max_bundle_iter) bdl = P->max_bundle_iter;
+        if (bdl > bdl_max) bdl_max = bdl;
+    }
+    if (bdl_max > 0) {
+        RUN(PH_OUTER_BEGIN, 0);
+        for (int b = 0; b < bdl_max; ++b) {
+            RUN(PH_BUNDLE_NEXT, b);
             RUN(108, 0);         /* QP -> gamma_test */
             RUN(107, 1);         /* L0 = L - B'(gamma_test) */
             RUN(101, 0);         /* IPM: L0 -> X_test */
@@ -326,16 +494,61 @@ int bqb_ctx_bound(bqb_ctx *c)
         RUN(107, 0);             /* L0 = L - B'(gamma) */
         RUN(101, 0);             /* IPM: L0 -> X_test */
         RUN(PH_REBUILD, 0);
-        bdl += count % 2;
-        if (bdl > P->max_bundle_iter) bdl = P->max_bundle_iter;
-        const int a = count_active(c);
-        if (a < 0) return -1;
-        if (a == 0) break;
+        if (fetch_states(c)) return -1;
     }
-    RUN(PH_FINISH, 0);
+    return collect(c);
+}
+
+int bqb_ctx_pop(bqb_ctx *c, long long *tag, double *bound, double *fracsol, int *xbest, double *LB_out,
+                int *improved, double *diff_out, int *status, int *stats)
+{
+    if (c->ndone == 0) return 0;
+    const struct done_node *e = &c->done[c->done_head];
+    const NodeState *s = &e->st;
+    const size_t B = (size_t)(c->d.n0 - 1);
+    if (tag) *tag = e->tag;
+    if (bound) *bound = s->bound;
+    if (fracsol) memcpy(fracsol, e->frac, sizeof(double) * B);
+    if (xbest) memcpy(xbest, e->xb, sizeof(int) * B);
+    if (LB_out) *LB_out = s->LB;
+    if (improved) *improved = s->improved;
+    if (diff_out) *diff_out = s->diff;
+    if (status) *status = s->status;
+    if (stats) {
+        stats[0] = s->prune; stats[1] = s->giveup; stats[2] = s->count; stats[3] = s->ipm_calls;
+        stats[4] = s->ipm_iters; stats[5] = s->max_cuts; stats[6] = s->max_bundle_seen; stats[7] = s->n;
+    }
+    c->done_head = (c->done_head + 1) % c->done_cap;
+    --c->ndone;
+    return 1;
+}
+
+/* Stage a batch into slots 0..nodes-1 and load it on the device (the basic
+ * bound has not run yet).  Discards anything in flight. */
+int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
+                 const double *diff, const int *is_root, const uint64_t *seeds)
+{
+    if (nodes < 1 || nodes > c->max_nodes) { bqb_set_error("batch of %d nodes (capacity %d)", nodes, c->max_nodes); return -1; }
+    engine_reset(c);
+    const size_t B = (size_t)(c->d.n0 - 1);
+    for (int i = 0; i < nodes; ++i)
+        if (bqb_ctx_push(c, i, xfixed + i * B, solX + i * B, LB ? LB[i] : 0.0, diff ? diff[i] : 0.0,
+                         is_root ? is_root[i] : 0, seeds ? seeds[i] : (uint64_t)i) < 0)
+            return -1;
+    if (upload_staged(c)) return -1;
+    c->loaded = 1;
     return 0;
 }
 
+/* Bound the loaded batch to the end (bounding.c:24-342 for every node). */
+int bqb_ctx_bound(bqb_ctx *c)
+{
+    while (bqb_ctx_busy(c) > 0)
+        if (bqb_ctx_advance(c)) return -1;
+    return 0;
+}
+
+/* Results of slots 0..nodes-1 as left on the device by the last batch. */
 int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, double *LB_out,
                     int *improved, double *diff_out, int *status, int *stats)
 {
@@ -361,20 +574,50 @@ int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, doub
     return 0;
 }
 
+/* Bound `count` nodes (any number) through the context's slots, refilling
+ * free slots after every outer round; outputs in input order. */
+int bqb_bound_stream(bqb_ctx *c, int count, const int *xfixed, const int *solX, const double *LB,
+                     const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
+                     double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
+                     int *status, int *stats)
+{
+    if (count < 0) { bqb_set_error("negative node count"); return -1; }
+    engine_reset(c);
+    const size_t B = (size_t)(c->d.n0 - 1);
+    float ms = 0.f;
+    hipEventRecord(c->ev0, c->stream);
+    int next = 0, got = 0;
+    while (got < count) {
+        while (next < count && c->nfree > 0) {
+            if (bqb_ctx_push(c, next, xfixed + next * B, solX + next * B, LB ? LB[next] : 0.0,
+                             diff ? diff[next] : 0.0, is_root ? is_root[next] : 0,
+                             seeds ? seeds[next] : (uint64_t)next) < 0)
+                return -1;
+            ++next;
+        }
+        if (bqb_ctx_advance(c)) return -1;
+        long long t;
+        while (bqb_ctx_pop(c, &t, bound ? bound + t : NULL, fracsol ? fracsol + t * B : NULL,
+                           xbest ? xbest + t * B : NULL, LB_out ? LB_out + t : NULL,
+                           improved ? improved + t : NULL, diff_out ? diff_out + t : NULL,
+                           status ? status + t : NULL, stats ? stats + 8 * t : NULL))
+            ++got;
+    }
+    hipEventRecord(c->ev1, c->stream);
+    if (bqb_check(hipEventSynchronize(c->ev1), "sync")) return -1;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    return 0;
+}
+
 int bqb_bound_batch(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
                     const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
                     double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
                     int *status, int *stats)
 {
-    if (bqb_ctx_load(c, nodes, xfixed, solX, LB, diff, is_root, seeds)) return -1;
-    float ms = 0.f;
-    hipEventRecord(c->ev0, c->stream);
-    if (bqb_ctx_bound(c)) return -1;
-    hipEventRecord(c->ev1, c->stream);
-    if (bqb_check(hipEventSynchronize(c->ev1), "sync")) return -1;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    c->last_ms = ms;
-    return bqb_ctx_results(c, bound, fracsol, xbest, LB_out, improved, diff_out, status, stats);
+    if (nodes < 1 || nodes > c->max_nodes) { bqb_set_error("batch of %d nodes (capacity %d)", nodes, c->max_nodes); return -1; }
+    return bqb_bound_stream(c, nodes, xfixed, solX, LB, diff, is_root, seeds, bound, fracsol, xbest,
+                            LB_out, improved, diff_out, status, stats);
 }
 
 /* fct_eval (src/bundle.c:242-275) of one matrix on node slot 0:
@@ -388,6 +631,7 @@ int bqb_fct_eval(bqb_ctx *c, int n, const double *L, int ntri, const int *tri, i
                  const int *pent, int nhept, const int *hept, const double *gamma, double *X,
                  double *g, double *f, int *status)
 {
+    if (bqb_ctx_busy(c) > 0) { bqb_set_error("fct_eval: the context has nodes in flight"); return -1; }
     const BatchDims d0 = c->d;
     const int ld = d0.ld, m = ntri + npent + nhept;
     if (n < 1 || n > ld) { bqb_set_error("fct_eval: n = %d outside 1..%d", n, ld); return -1; }

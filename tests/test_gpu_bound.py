@@ -342,3 +342,27 @@ def test_dropin_fct_eval_without_cuts_is_the_basic_bound():
     _, _, phi, _, _ = O.ipm(PL)
     assert abs(f - phi) <= 1e-6 * abs(phi)
     assert np.allclose(np.diag(X), 1.0, atol=1e-9)
+
+
+def test_stream_through_few_slots_equals_batch_bitwise():
+    """Slots are refilled after every outer round: 40 nodes through 7 slots
+    give exactly the results of one 40-node batch (a node's bound depends
+    only on its own inputs, not on its slot or its neighbours)."""
+    n0 = 45
+    SP = O.build_L(I.gnp_adj(n0, 0.4, 5, weights="pm1"))
+    P = api.make_params()
+    nodes = 40
+    xf, sx = I.random_fixings(n0, 20, nodes, 6)
+    seeds = np.arange(7, 7 + nodes, dtype=np.uint64)
+    root = api.BoundContext(SP, params=P, max_nodes=1)
+    z = np.zeros((1, n0 - 1), np.int32)
+    r0 = root.bound(z, z, LB=0.0, is_root=1, seeds=1)
+    LB, diff = r0["LB"][0], r0["diff"][0]
+    big = api.BoundContext(SP, params=P, max_nodes=nodes)
+    a = big.bound(xf, sx, LB=LB, diff=diff, seeds=seeds)
+    small = api.BoundContext(SP, params=P, max_nodes=7)
+    b = small.bound(xf, sx, LB=LB, diff=diff, seeds=seeds, stream=True)
+    assert np.all(a["status"] == 0)
+    for k in ("bound", "fracsol", "xbest", "LB", "improved", "diff", "stats"):
+        assert np.array_equal(a[k], b[k]), k
+    assert len(set(a["stats"][:, 2])) > 1          # the nodes ran different numbers of rounds

@@ -584,25 +584,41 @@ int bqb_bound_stream(bqb_ctx *c, int count, const int *xfixed, const int *solX, 
     if (count < 0) { bqb_set_error("negative node count"); return -1; }
     engine_reset(c);
     const size_t B = (size_t)(c->d.n0 - 1);
+    double *frac = malloc(sizeof(double) * (B ? B : 1));
+    int *xb = malloc(sizeof(int) * (B ? B : 1));
+    if (!frac || !xb) { free(frac); free(xb); bqb_set_error("calloc"); return -1; }
     float ms = 0.f;
     hipEventRecord(c->ev0, c->stream);
-    int next = 0, got = 0;
+    int next = 0, got = 0, rc = -1;
     while (got < count) {
         while (next < count && c->nfree > 0) {
             if (bqb_ctx_push(c, next, xfixed + next * B, solX + next * B, LB ? LB[next] : 0.0,
                              diff ? diff[next] : 0.0, is_root ? is_root[next] : 0,
                              seeds ? seeds[next] : (uint64_t)next) < 0)
-                return -1;
+                goto out;
             ++next;
         }
-        if (bqb_ctx_advance(c)) return -1;
+        if (bqb_ctx_advance(c)) goto out;
         long long t;
-        while (bqb_ctx_pop(c, &t, bound ? bound + t : NULL, fracsol ? fracsol + t * B : NULL,
-                           xbest ? xbest + t * B : NULL, LB_out ? LB_out + t : NULL,
-                           improved ? improved + t : NULL, diff_out ? diff_out + t : NULL,
-                           status ? status + t : NULL, stats ? stats + 8 * t : NULL))
+        double bd, lbo, dfo;
+        int imp, st, sts[8];
+        while (bqb_ctx_pop(c, &t, &bd, frac, xb, &lbo, &imp, &dfo, &st, sts)) {
+            if (bound) bound[t] = bd;
+            if (fracsol) memcpy(fracsol + t * B, frac, sizeof(double) * B);
+            if (xbest) memcpy(xbest + t * B, xb, sizeof(int) * B);
+            if (LB_out) LB_out[t] = lbo;
+            if (improved) improved[t] = imp;
+            if (diff_out) diff_out[t] = dfo;
+            if (status) status[t] = st;
+            if (stats) memcpy(stats + 8 * t, sts, sizeof sts);
             ++got;
+        }
     }
+    rc = 0;
+out:
+    free(frac);
+    free(xb);
+    if (rc) return rc;
     hipEventRecord(c->ev1, c->stream);
     if (bqb_check(hipEventSynchronize(c->ev1), "sync")) return -1;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);

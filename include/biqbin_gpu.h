@@ -259,6 +259,12 @@ int bqb_ipm_debug(int n, const double *L, int max_iters, double *X, double *Zi, 
                   double *dX, double *y, double *phi, int *iters, double *alpha_trace,
                   double *dbg_out);
 
+/* Diagnostic (no counterpart in the reference): copy `bytes` (multiple of
+ * 16) between two device buffers `reps` times with a plain streaming kernel
+ * (copy_probe_kernel); *ms = average time per copy.  Calibrates HBM byte
+ * counters: each copy moves exactly 2 x bytes. */
+int bqb_copy_probe(size_t bytes, int reps, double *ms);
+
 /* Diagnostic: apply one workgroup-distributed primitive to an n x n matrix
  * (column-major) with register tile `tile` (1604, 1607 or 3204 = P*100 + R):
  * op 0 inverse by symmetric sweep, 1 lower Cholesky factor, 2 Cholesky

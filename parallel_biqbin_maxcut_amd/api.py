@@ -15,6 +15,7 @@ def declare(L):
     L.bqb_ipm_debug.restype = C.c_int
     L.bqb_dist_selftest.argtypes = [C.c_int, C.c_int, C.c_int, N.dptr, N.dptr, N.iptr]
     L.bqb_dist_selftest.restype = C.c_int
+    L.bqb_copy_probe.argtypes = [C.c_size_t, C.c_int, N.dptr]
     L.bqb_ipm_ld_for.argtypes = [C.c_int]
     L.bqb_ipm_ld_for.restype = C.c_int
     _declare_ctx(L)
@@ -89,6 +90,15 @@ def dist_selftest(tile, op, A):
     ok = C.c_int()
     N.check(N.lib().bqb_dist_selftest(tile, op, n, _d(A), _d(out), C.byref(ok)))
     return out, ok.value
+
+
+def copy_probe(nbytes=256 << 20, reps=10):
+    """Average ms of a plain device copy of nbytes (HBM counter calibration:
+    each copy moves 2 x nbytes)."""
+    ms = C.c_double()
+    N.check(N.lib().bqb_copy_probe(nbytes, reps, C.byref(ms)))
+    return ms.value
+
 
 
 # ---------------------------------------------------------------- batch bound
@@ -397,3 +407,11 @@ class BoundContext:
     def matrix(self, buf, node, n):
         a = self.get(buf, node, np.float64, self.ld * self.ld).reshape((self.ld, self.ld), order="F")
         return np.array(a[:n, :n], order="F")
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "copy-probe":
+        nb = int(sys.argv[2]) if len(sys.argv) > 2 else 256 << 20
+        ms = copy_probe(nb, 10)
+        print(f"copy of {nb} bytes: {ms:.4f} ms, {2 * nb / ms / 1e6:.1f} GB/s")

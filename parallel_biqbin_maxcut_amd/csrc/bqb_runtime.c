@@ -278,3 +278,35 @@ out:
     free(h);
     return rc;
 }
+
+hipError_t bqb_launch_copy_probe(const void *src, void *dst, size_t bytes, int reps);
+
+/* Copies `bytes` (a multiple of 16) between two device buffers `reps` times
+ * with a plain streaming kernel; *ms receives the average time per copy.
+ * Calibration of the HBM byte counters (each copy moves 2 x bytes). */
+int bqb_copy_probe(size_t bytes, int reps, double *ms)
+{
+    if (bqb_ensure_init()) return -1;
+    if (bytes == 0 || bytes % 16 || reps < 1) { bqb_set_error("copy probe: bytes must be a positive multiple of 16"); return -1; }
+    void *a = NULL, *b = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    int rc = -1;
+    if (bqb_check(hipMalloc(&a, bytes), "hipMalloc") || bqb_check(hipMalloc(&b, bytes), "hipMalloc")) goto out;
+    if (bqb_check(hipMemset(a, 1, bytes), "hipMemset")) goto out;
+    if (bqb_check(hipEventCreate(&e0), "event") || bqb_check(hipEventCreate(&e1), "event")) goto out;
+    if (bqb_check(bqb_launch_copy_probe(a, b, bytes, 1), "copy probe")) goto out;   /* warm-up */
+    hipEventRecord(e0, 0);
+    if (bqb_check(bqb_launch_copy_probe(a, b, bytes, reps), "copy probe")) goto out;
+    hipEventRecord(e1, 0);
+    if (bqb_check(hipEventSynchronize(e1), "sync")) goto out;
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t / reps;
+    rc = 0;
+out:
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
+}

@@ -29,7 +29,25 @@ __global__ __launch_bounds__(P * P) void dist_selftest_kernel(int op, int n, int
     if (t.tid == 0) *flag = ok ? 1 : 0;
 }
 
+// Plain streaming copy (16-byte vectors, grid-stride): the calibration kernel
+// for the HBM byte counters (its algorithmic traffic is exactly 2 x bytes).
+__global__ __launch_bounds__(256) void copy_probe_kernel(const double2 *__restrict__ src,
+                                                         double2 *__restrict__ dst, size_t n2)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 }  // namespace bqb
+
+extern "C" hipError_t bqb_launch_copy_probe(const void *src, void *dst, size_t bytes, int reps)
+{
+    const size_t n2 = bytes / sizeof(double2);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(bqb::copy_probe_kernel, dim3(4096), dim3(256), 0, 0, (const double2 *)src,
+                           (double2 *)dst, n2);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t bqb_launch_dist_selftest(int tile, int op, int n, int ld, const double *in,
                                                double *out, int *flag)

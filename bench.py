@@ -14,8 +14,8 @@ is refilled after each outer round, as the B&B node queue runs it).  Inputs
 are uploaded inside the call (a few hundred KB; the timed region includes
 it).  value = node bounds per second.  The same nodes bounded as one fixed
 512-node batch (every launch waits for the slowest node) are reported in
-config.single_batch_bounds_per_s.  The roofline object covers the dominant kernel, the batched
-IPM: algorithmic FP64 flops (6 n^3 per iteration + n^3/3 per line-search
+config.single_batch_bounds_per_s.  The roofline object covers the dominant
+kernel, the batched IPM: algorithmic FP64 flops (6 n^3 per iteration + n^3/3 per line-search
 Cholesky test, counted on the device) over the summed duration of its launches
 (HIP event pairs on the launch stream, during the timed steps).  The
 cpu_baseline times the CPU oracle (the FP64 restatement of the same
@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-batch comparison after the timed steps (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build bounds on one GPU; --gpus {args.gpus} is not supported "
@@ -120,11 +122,13 @@ def main():
         raise SystemExit(f"node bound failures: {np.unique(r['status'], return_counts=True)}")
 
     # the first 512 nodes as one fixed batch (all launches wait for its slowest node)
-    tb = time.perf_counter()
-    rb = ctx.bound(xf[: args.nodes], sx[: args.nodes], LB=LB, diff=diff, is_root=0, seeds=seeds[: args.nodes])
-    single_batch = args.nodes / (time.perf_counter() - tb)
-    if not np.array_equal(rb["bound"], r["bound"][: args.nodes]):
-        raise SystemExit("streamed and batched bounds differ")
+    single_batch = None
+    if not args.no_single:
+        tb = time.perf_counter()
+        rb = ctx.bound(xf[: args.nodes], sx[: args.nodes], LB=LB, diff=diff, is_root=0, seeds=seeds[: args.nodes])
+        single_batch = round(args.nodes / (time.perf_counter() - tb), 2)
+        if not np.array_equal(rb["bound"], r["bound"][: args.nodes]):
+            raise SystemExit("streamed and batched bounds differ")
 
     ms_per_step = dt / args.steps * 1e3
     value = args.per_step * args.steps / dt
@@ -149,7 +153,7 @@ def main():
                         "with triangle/pentagonal/heptagonal cuts",
             "nodes_per_launch": args.nodes,
             "nodes_per_step": args.per_step,
-            "single_batch_bounds_per_s": round(single_batch, 2),
+            "single_batch_bounds_per_s": single_batch,
             "seed": args.seed,
             "incumbent": LB,
             "root_gap": round(diff, 4),

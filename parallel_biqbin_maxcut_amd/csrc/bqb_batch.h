@@ -41,6 +41,8 @@ typedef struct NodeState {
     int off_pent, off_hept;   /* gamma offsets of the pentagonal / heptagonal
                                  multipliers before this round's purge */
     int stage;          /* slot life cycle: BQB_STAGE_* */
+    int bdl_left;       /* bundle iterations left in the current outer round */
+    int rphase;         /* outer round: 0 to begin, 1 bundle iterations, 2 round end */
     double fixedvalue, f, f_test, t, bound, basic_bound, oldf;
     double LB, diff, viol3, viol5, gap, del, lmax;
     uint64_t seed;
@@ -65,6 +67,7 @@ typedef struct BatchDims {
     int kg_list;        /* pentagonal/heptagonal list capacity */
     int max_contrib;    /* op_Bt gather capacity */
     int npairs;         /* ld*(ld-1)/2 */
+    int sa_chains5, sa_chains7;   /* SA chains per pentagonal / heptagonal round */
 } BatchDims;
 
 /* Device pointers of a batch (all arrays are per node at the stated stride). */

@@ -388,6 +388,8 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
         sp->serious = serious;
         sp->ipm_calls = s.ipm_calls + 1;
         sp->ipm_iters = s.ipm_iters + p.ipm_iters[node];
+        sp->bdl_left = s.bdl_left - 1;
+        if (sp->bdl_left <= 0) sp->rphase = 2;
     }
     (void)s_keep;
 }

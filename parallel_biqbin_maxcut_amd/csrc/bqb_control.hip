@@ -201,14 +201,24 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         case PH_OUTER_C: case PH_REBUILD: case PH_BUNDLE_NEXT: want = BQB_STAGE_LOOP; break;
         default: break;
         }
-        if (want >= 0 && s.stage != want) {
+        // outer rounds advance per node: begin (rphase 0), bundle
+        // iterations (1), round end (2)
+        int want_r = -1;
+        switch (phase) {
+        case PH_OUTER_BEGIN: want_r = 0; break;
+        case PH_BUNDLE_NEXT: want_r = 1; break;
+        case PH_OUTER_A: case PH_OUTER_B: case PH_SEP_PENT: case PH_SEP_HEPT: case PH_OUTER_C:
+        case PH_REBUILD: want_r = 2; break;
+        default: break;
+        }
+        if ((want >= 0 && s.stage != want) || (want_r >= 0 && s.rphase != want_r)) {
             if (threadIdx.x == 0) p.mask[node] = 0;
             return;
         }
     }
 
     if (phase == PH_BUNDLE_NEXT) {
-        if (threadIdx.x == 0) p.mask[node] = (s.active && arg < s.bdl_iter) ? 1 : 0;
+        if (threadIdx.x == 0) p.mask[node] = (s.active && s.bdl_left > 0) ? 1 : 0;
         return;
     }
 
@@ -269,6 +279,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             sp->t = 0.5 * (s.bound - s.LB) / (m * s.viol3 * s.viol3);
             sp->cuts_changed = 1;
             sp->stage = BQB_STAGE_LOOP;
+            sp->rphase = 0;
             p.mask[node] = 1;
         }
         return;
@@ -285,6 +296,8 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
                 sp->count = c;
                 sp->oldf = s.f;
                 sp->bdl_iter = bdl < P.max_bundle_iter ? bdl : P.max_bundle_iter;
+                sp->bdl_left = sp->bdl_iter;
+                sp->rphase = sp->bdl_left > 0 ? 1 : 2;
             }
             p.mask[node] = s.active;
         }
@@ -415,6 +428,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             sp->ipm_calls += 1;
             sp->ipm_iters += p.ipm_iters[node];
             if (p.ipm_status[node]) { sp->status = p.ipm_status[node]; sp->active = 0; }
+            sp->rphase = 0;
         }
         return;
     }

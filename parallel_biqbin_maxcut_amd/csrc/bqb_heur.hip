@@ -49,6 +49,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     __shared__ double s_col[2 * NMAX];
     __shared__ double s_xh[NMAX];
     __shared__ signed char s_sign[GW_BATCH][NMAX];
+    __shared__ double s_v[GW_BATCH][NMAX];        // this batch's hyperplanes
     __shared__ signed char s_lift[GW_BATCH][NMAX_GW];
     __shared__ int s_sol[NMAX_GW];
     __shared__ double s_val[GW_BATCH];
@@ -99,9 +100,10 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     double fh;
     {
         double acc = 0.0;
-        for (int e = t.tid; e < B * B; e += NT) {
-            const int i = e / B, j = e % B;
-            if (s_sol[i] && s_sol[j]) acc += p.SP_L[j + (size_t)i * n0];
+        for (int i = wave; i < B; i += NW) {
+            if (!s_sol[i]) continue;
+            for (int j = lane; j < B; j += 64)
+                if (s_sol[j]) acc += p.SP_L[j + (size_t)i * n0];
         }
         fh = wg_sum<P>(acc, s_red, t);
     }
@@ -124,15 +126,18 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         // ---- GW_heuristic (heuristic.c:119-186): n0 hyperplanes ----------
         for (int h0 = 0; h0 < n0; h0 += GW_BATCH) {
             const int hb = min(GW_BATCH, n0 - h0);
-            // sign(Z v) per (hyperplane, row), products summed in j order
+            // the batch's random vectors v_j = U - 1/2, drawn once
             for (int e = t.tid; e < hb * n; e += NT) {
-                const int hh = e / n, i = e % n;
-                const uint64_t base = (uint64_t)(h0 + hh) * BQB_GW_STRIDE;
+                const int hh = e / n, j = e - hh * n;
+                s_v[hh][j] = bqb_u01(bqb_draw(rkey, (uint64_t)(h0 + hh) * BQB_GW_STRIDE + j)) - 0.5;
+            }
+            __syncthreads();
+            // sign(Z v) per (hyperplane, row), products summed in j order
+            // without fused multiply-add (as the reference's scalar loop)
+            for (int e = t.tid; e < hb * n; e += NT) {
+                const int hh = e / n, i = e - hh * n;
                 double sca = 0.0;
-                for (int j = 0; j <= i; ++j) {
-                    const double v = bqb_u01(bqb_draw(rkey, base + j)) - 0.5;
-                    sca = __dadd_rn(sca, __dmul_rn(v, Zs[i + (size_t)j * ld]));
-                }
+                for (int j = 0; j <= i; ++j) sca = __dadd_rn(sca, __dmul_rn(s_v[hh][j], Zs[i + (size_t)j * ld]));
                 s_sign[hh][i] = (sca < 0) ? -1 : 1;
             }
             __syncthreads();
@@ -140,14 +145,18 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
             // stays in LDS, Lx in registers (row i = lane + 64 q)
             for (int hh = wave; hh < hb; hh += NW) {
                 signed char *xcur = s_sign[hh];
+                // Lx_i = sum_j L(i, j) x_j in j order; L is symmetric, so
+                // column j is read (lanes on consecutive rows)
                 double Lx[PL];
 #pragma unroll
-                for (int q = 0; q < PL; ++q) {
-                    const int i = lane + 64 * q;
-                    double acc = 0.0;
-                    if (i < n)
-                        for (int j = 0; j < n; ++j) acc += L[j + (size_t)i * ld] * (double)xcur[j];
-                    Lx[q] = acc;
+                for (int q = 0; q < PL; ++q) Lx[q] = 0.0;
+                for (int j = 0; j < n; ++j) {
+                    const double xj = (double)xcur[j];
+#pragma unroll
+                    for (int q = 0; q < PL; ++q) {
+                        const int i = lane + 64 * q;
+                        if (i < n) Lx[q] += L[i + (size_t)j * ld] * xj;
+                    }
                 }
                 double bv = -1e9;
                 int bi = 0x7fffffff;
@@ -169,7 +178,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                     for (int q = 0; q < PL; ++q) {
                         const int i = lane + 64 * q;
                         if (i < n) {
-                            const double lij = L[index + (size_t)i * ld];
+                            const double lij = L[i + (size_t)index * ld];   // = L(index, i)
                             if (fabs(lij) > 0.001) Lx[q] = (xi > 0) ? Lx[q] - 2 * lij : Lx[q] + 2 * lij;
                         }
                     }
@@ -193,11 +202,13 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                 __builtin_amdgcn_wave_barrier();
                 for (int a = lane; a < n - 1; a += 64) s_lift[hh][s_free[a]] = (signed char)((xcur[a] + 1) / 2);
                 __builtin_amdgcn_wave_barrier();
-                // its cut value (update_best -> evaluateSolution)
+                // its cut value (update_best -> evaluateSolution): rows with
+                // sol_i = 1, lanes over j; integer data, so the sum is exact
                 double acc = 0.0;
-                for (int e = lane; e < B * B; e += 64) {
-                    const int i = e / B, j = e % B;
-                    if (s_lift[hh][i] && s_lift[hh][j]) acc += p.SP_L[j + (size_t)i * n0];
+                for (int i = 0; i < B; ++i) {
+                    if (!s_lift[hh][i]) continue;
+                    for (int j = lane; j < B; j += 64)
+                        if (s_lift[hh][j]) acc += p.SP_L[j + (size_t)i * n0];
                 }
                 acc = wave_sum(acc);
                 if (lane == 0) s_val[hh] = acc;
@@ -259,10 +270,13 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         if (t.tid == 0) { sp->status = status; sp->active = 0; p.mask[node] = 0; }
         return;
     }
+    for (int i = t.tid; i < B; i += NT) s_sol[i] = xo[i];
+    __syncthreads();
     double acc = 0.0;
-    for (int e = t.tid; e < B * B; e += NT) {
-        const int i = e / B, j = e % B;
-        if (xo[i] && xo[j]) acc += p.SP_L[j + (size_t)i * n0];
+    for (int i = wave; i < B; i += NW) {
+        if (!s_sol[i]) continue;
+        for (int j = lane; j < B; j += 64)
+            if (s_sol[j]) acc += p.SP_L[j + (size_t)i * n0];
     }
     acc = wg_sum<P>(acc, s_red, t);
     if (acc > s.LB) {

@@ -21,20 +21,21 @@ namespace bqb {
 constexpr int ST = 256;
 constexpr int SA_MAX_STAGES = 150;
 
-__device__ __forceinline__ int pk_get(const int (&pk)[7], int i, int k)
-{
-    int r = pk[0];
-#pragma unroll
-    for (int j = 1; j < 7; ++j)
-        if (j < k && i == j) r = pk[j];
-    return r;
-}
-__device__ __forceinline__ void pk_set(int (&pk)[7], int i, int v)
-{
-#pragma unroll
-    for (int j = 0; j < 7; ++j)
-        if (i == j) pk[j] = v;
-}
+// The first K entries of a chain's permutation, 16 bits each in two 64-bit
+// words (entries 0-3, 4-6): read and written at run-time positions by shifts,
+// so they stay in registers.
+struct Head {
+    uint64_t w0, w1;
+    __device__ __forceinline__ int get(int i) const
+    {
+        return (int)(((i < 4) ? (w0 >> (16 * i)) : (w1 >> (16 * (i - 4)))) & 0xffffull);
+    }
+    __device__ __forceinline__ void set(int i, int v)
+    {
+        if (i < 4) w0 = (w0 & ~(0xffffull << (16 * i))) | ((uint64_t)v << (16 * i));
+        else w1 = (w1 & ~(0xffffull << (16 * (i - 4)))) | ((uint64_t)v << (16 * (i - 4)));
+    }
+};
 
 // packed symmetric X: element (a, b) at max*(max+1)/2 + min
 __device__ __forceinline__ double xs(const double *Xp, int a, int b)
@@ -43,10 +44,24 @@ __device__ __forceinline__ double xs(const double *Xp, int a, int b)
     return Xp[hi * (hi + 1) / 2 + lo];
 }
 
-template <int K>
+// A chain's permutation: in LDS (16-bit entries) when it fits next to the
+// packed X, otherwise in the node's global scratch.
+template <bool LDSP>
+struct PermRef {
+    unsigned short *s;
+    int *g;
+    __device__ __forceinline__ int get(int i) const { return LDSP ? (int)s[i] : g[i]; }
+    __device__ __forceinline__ void set(int i, int v) const
+    {
+        if (LDSP) s[i] = (unsigned short)v;
+        else g[i] = v;
+    }
+};
+
+template <int K, bool LDSP>
 __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
 {
-    extern __shared__ double s_X[];            // packed triangle of X
+    extern __shared__ double s_X[];            // packed triangle of X, then the LDS permutations
     __shared__ double s_t0;
     __shared__ int s_cnt[ST];
     __shared__ int s_qn;
@@ -96,15 +111,19 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
         const int type = c % NTYPES + 1;
         const uint64_t key = bqb_key(rkey, (uint64_t)c);
         uint64_t ctr = 0;
-        int *perm = p.perm + ((size_t)node * BQB_SA_CHAINS + c) * ld;
-        for (int i = 0; i < n; ++i) perm[i] = i;
+        PermRef<LDSP> perm;
+        perm.s = reinterpret_cast<unsigned short *>(s_X + (size_t)ld * (ld + 1) / 2) + (size_t)c * ld;
+        perm.g = p.perm + ((size_t)node * BQB_SA_CHAINS + c) * ld;
+        for (int i = 0; i < n; ++i) perm.set(i, i);
         for (int i = n - 1; i > 0; --i) {
             const int r = bqb_rmod(bqb_draw(key, ctr++), i);
-            const int tmp = perm[i]; perm[i] = perm[r]; perm[r] = tmp;
+            const int tmp = perm.get(i);
+            perm.set(i, perm.get(r));
+            perm.set(r, tmp);
         }
-        int pk[7], pb[7];
+        Head pk = {0ull, 0ull}, pb = {0ull, 0ull};
 #pragma unroll
-        for (int j = 0; j < 7; ++j) { pk[j] = (j < K) ? perm[j] : 0; pb[j] = 0; }
+        for (int j = 0; j < K; ++j) pk.set(j, perm.get(j));
         int e[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) e[j] = kgon_e(type, j);
@@ -114,7 +133,7 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
 #pragma unroll
         for (int i = 0; i < K; ++i)
 #pragma unroll
-            for (int j = i + 1; j < K; ++j) sol += 2 * (e[j] * e[i]) * xs(s_X, pk[j], pk[i]);
+            for (int j = i + 1; j < K; ++j) sol += 2 * (e[j] * e[i]) * xs(s_X, pk.get(j), pk.get(i));
         double cost = 1e9;
         int not_done = 1, stages = 0, status = 0;
         while (not_done > 0) {
@@ -124,18 +143,18 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
                 int i1 = bqb_rmod(bqb_draw(key, ctr++), K);
                 int i2 = bqb_rmod(bqb_draw(key, ctr++), n);
                 if (i2 < i1) { const int tmp = i1; i1 = i2; i2 = tmp; }
-                const int p1 = pk_get(pk, i1, K);
-                const int p2 = (i2 < K) ? pk_get(pk, i2, K) : perm[i2];
+                const int p1 = pk.get(i1);
+                const int p2 = (i2 < K) ? pk.get(i2) : perm.get(i2);
                 const int ei1 = kgon_e(type, i1);
                 double delta = 0.0;
 #pragma unroll
                 for (int i = 0; i < K; ++i)
-                    delta += (double)(e[i] * ei1) * (xs(s_X, pk[i], p2) - xs(s_X, pk[i], p1));
+                    delta += (double)(e[i] * ei1) * (xs(s_X, pk.get(i), p2) - xs(s_X, pk.get(i), p1));
                 if (i2 < K) {
                     const int ei2 = kgon_e(type, i2);
 #pragma unroll
                     for (int i = 0; i < K; ++i)
-                        delta += (double)(e[i] * ei2) * (xs(s_X, pk[i], p1) - xs(s_X, pk[i], p2));
+                        delta += (double)(e[i] * ei2) * (xs(s_X, pk.get(i), p1) - xs(s_X, pk.get(i), p2));
                 }
                 delta *= 2.0;
                 const double tv1 = 2.0 - 2.0 * xs(s_X, p2, p1);
@@ -152,14 +171,13 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
                     accept = 1;
                 if (accept) {
                     if (fabs(delta) > 0.001) not_done = 1;
-                    pk_set(pk, i1, p2);
-                    if (i2 < K) pk_set(pk, i2, p1);
-                    else perm[i2] = p1;
+                    pk.set(i1, p2);
+                    if (i2 < K) pk.set(i2, p1);
+                    else perm.set(i2, p1);
                     sol = sol_temp;
                     if (sol < cost) {
                         cost = sol;
-#pragma unroll
-                        for (int j = 0; j < 7; ++j) pb[j] = pk[j];
+                        pb = pk;
                     }
                 }
             }
@@ -167,7 +185,8 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
             m1 = (int)round(m1 * 1.1);
         }
         p.sa_val[(size_t)node * BQB_SA_CHAINS + c] = cost;
-        for (int j = 0; j < K; ++j) p.sa_perm[((size_t)node * BQB_SA_CHAINS + c) * 7 + j] = pb[j];
+#pragma unroll
+        for (int j = 0; j < K; ++j) p.sa_perm[((size_t)node * BQB_SA_CHAINS + c) * 7 + j] = pb.get(j);
         p.sa_status[(size_t)node * BQB_SA_CHAINS + c] = status;
     }
     __syncthreads();
@@ -283,8 +302,20 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
 
 extern "C" hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStream_t s)
 {
-    const size_t lds = sizeof(double) * (size_t)d->ld * (d->ld + 1) / 2;
-    if (k == 5) hipLaunchKernelGGL(bqb::sa_kernel<5>, dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
-    else hipLaunchKernelGGL(bqb::sa_kernel<7>, dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+    const size_t lds_x = sizeof(double) * (size_t)d->ld * (d->ld + 1) / 2;
+    const int chains = (k == 5) ? d->sa_chains5 : d->sa_chains7;
+    const size_t lds_p = sizeof(unsigned short) * (size_t)chains * d->ld;
+    // the chains' permutations join the packed X in LDS while both fit in
+    // the 160 KB of a CU (less the kernel's static LDS); otherwise they stay
+    // in the node's global scratch
+    const bool in_lds = lds_x + lds_p <= 156 * 1024;
+    const size_t lds = in_lds ? lds_x + lds_p : lds_x;
+    if (k == 5) {
+        if (in_lds) hipLaunchKernelGGL((bqb::sa_kernel<5, true>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+        else hipLaunchKernelGGL((bqb::sa_kernel<5, false>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+    } else {
+        if (in_lds) hipLaunchKernelGGL((bqb::sa_kernel<7, true>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+        else hipLaunchKernelGGL((bqb::sa_kernel<7, false>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+    }
     return hipGetLastError();
 }

@@ -40,7 +40,8 @@ struct bqb_ctx {
     BatchPtrs p;
     BqbParams params;
     int max_nodes;
-    int tile;
+    int tile;           /* IPM register tile (P*100 + R) */
+    int heur_tile;      /* heuristic tile: 1024 threads where they fit */
     void *arena;
     size_t arena_bytes;
     hipStream_t stream;
@@ -116,6 +117,10 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     if (max_hept <= 0) max_hept = 2048;
     int tile = bqb_ipm_tile_for(n0);
     if (!tile) { bqb_set_error("n0 = %d exceeds the largest compiled tile", n0); return NULL; }
+    /* the heuristic's 1-opt runs one hyperplane per wave: 16 waves (32 x 32
+     * threads) where the 32 x R tile covers n0 */
+    const int heur_tile = (n0 <= 128) ? 3204 : 3205;
+    const int tile_ld = (tile / 100) * (tile % 100), heur_ld = (heur_tile / 100) * (heur_tile % 100);
     bqb_ctx *c = calloc(1, sizeof(bqb_ctx));
     if (!c) { bqb_set_error("calloc"); return NULL; }
     c->params = *params;
@@ -132,10 +137,11 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     }
     c->max_nodes = max_nodes;
     c->tile = tile;
+    c->heur_tile = heur_tile;
     BatchDims *d = &c->d;
     d->nodes = max_nodes;
     d->n0 = n0;
-    d->ld = (tile / 100) * (tile % 100);
+    d->ld = tile_ld > heur_ld ? tile_ld : heur_ld;   /* slabs fit both tiles */
     d->max_bundle = max_bundle;
     d->max_tri = max_tri;
     d->max_pent = max_pent;
@@ -145,6 +151,8 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     d->kg_list = BQB_SA_CHAINS;
     d->max_contrib = 3 * max_tri + 10 * max_pent + 21 * max_hept;
     d->npairs = d->ld * (d->ld - 1) / 2;
+    d->sa_chains5 = 3 * c->params.Pent_Trials;
+    d->sa_chains7 = 4 * c->params.Hepta_Trials;
 
     const size_t N = (size_t)max_nodes, slab = (size_t)d->ld * d->ld, B = (size_t)(n0 - 1);
     struct { void **ptr; size_t bytes; } items[] = {
@@ -304,7 +312,7 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     switch (what) {
     case 100: return ipm_launch(c, p->L, p->X);
     case 101: return ipm_launch(c, p->L0, p->Xt);
-    case 102: return bqb_check(bqb_launch_heur(d, p, c->tile, s), "heur launch");
+    case 102: return bqb_check(bqb_launch_heur(d, p, c->heur_tile, s), "heur launch");
     case 103: return bqb_check(bqb_launch_trisep(d, p, s), "trisep launch");
     case 104: return bqb_check(bqb_launch_sa(d, p, 5, s), "sa5 launch");
     case 105: return bqb_check(bqb_launch_sa(d, p, 7, s), "sa7 launch");
@@ -464,17 +472,29 @@ int bqb_ctx_advance(bqb_ctx *c)
         c->loaded = 0;
     }
     if (fetch_states(c)) return -1;
-    int bdl_max = 0;
+    /* one slice of the outer loop: every node in the loop begins a round if
+     * it is between rounds, runs up to init_bundle_iter of its bundle
+     * iterations, and goes through the round end once its bundle iterations
+     * are done (a round longer than the slice continues in the next call) */
+    int in_loop = 0, bmax = 0;
+    const int slice = P->init_bundle_iter > 0 ? P->init_bundle_iter : 1;
     for (int i = 0; i < c->hi; ++i) {
         const NodeState *s = &c->h_st[i];
         if (c->tag[i] < 0 || !s->active || s->stage != BQB_STAGE_LOOP) continue;
-        int bdl = P->init_bundle_iter + (s->count + 1) / 2;
-        if (bdl > P->max_bundle_iter) bdl = P->max_bundle_iter;
-        if (bdl > bdl_max) bdl_max = bdl;
+        in_loop = 1;
+        int left = s->bdl_left;
+        if (s->rphase == 0) {
+            left = P->init_bundle_iter + (s->count + 1) / 2;
+            if (left > P->max_bundle_iter) left = P->max_bundle_iter;
+        } else if (s->rphase == 2) {
+            left = 0;
+        }
+        if (left > bmax) bmax = left;
     }
-    if (bdl_max > 0) {
+    if (in_loop) {
+        const int nb = bmax < slice ? bmax : slice;
         RUN(PH_OUTER_BEGIN, 0);
-        for (int b = 0; b < bdl_max; ++b) {
+        for (int b = 0; b < nb; ++b) {
             RUN(PH_BUNDLE_NEXT, b);
             RUN(108, 0);         /* QP -> gamma_test */
             RUN(107, 1);         /* L0 = L - B'(gamma_test) */

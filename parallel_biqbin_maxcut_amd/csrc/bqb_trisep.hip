@@ -18,7 +18,7 @@
 namespace bqb {
 
 constexpr int TT = 256;            // threads
-constexpr int PER = 4;             // triples per thread per chunk
+constexpr int PER = 8;             // consecutive triples per thread per chunk
 constexpr int CHUNK = TT * PER;
 constexpr int TRI_LIST_MAX = BQB_TRI_LIST_MAX;   // LDS list capacity
 
@@ -45,6 +45,7 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
     __shared__ double s_red[TT / 64];
     __shared__ int s_size, s_least, s_qn;
     __shared__ double s_leastv;
+    __shared__ int s_any[2][TT / 64];   // per-wave "chunk has a violation", double buffered
 
     const int node = blockIdx.x;
     if (node >= d.nodes || !p.mask[node]) return;
@@ -93,27 +94,34 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
     double vmax = -1e9;
     const long long per_type = c3(n);
     const long long total = 4 * per_type;
-    for (long long q0 = 0; q0 < total; q0 += CHUNK) {
+    int parity = 0;
+    for (long long q0 = 0; q0 < total; q0 += CHUNK, parity ^= 1) {
         double v[PER];
         unsigned id[PER];
         int nv = 0;
+        // decode the first of this thread's PER consecutive triples, then step
+        // through (type, ii > jj > kk) in scan order
+        const long long qs = q0 + (long long)threadIdx.x * PER;
+        int type = 1, ii = 2, jj = 1, kk = 0;
+        if (qs < total) {
+            type = (int)(qs / per_type) + 1;
+            long long r = qs - (long long)(type - 1) * per_type;
+            // ii: largest with C(ii,3) <= r
+            ii = (int)cbrt(6.0 * (double)r) + 1;
+            while (ii > 2 && c3(ii) > r) --ii;
+            while (c3(ii + 1) <= r) ++ii;
+            r -= c3(ii);
+            jj = (int)((1.0 + sqrt(1.0 + 8.0 * (double)r)) * 0.5);
+            while (jj > 1 && (long long)jj * (jj - 1) / 2 > r) --jj;
+            while ((long long)(jj + 1) * jj / 2 <= r) ++jj;
+            kk = (int)(r - (long long)jj * (jj - 1) / 2);
+        }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const long long q = q0 + (long long)threadIdx.x * PER + u;
+            const long long q = qs + u;
             v[u] = -1e300;
             id[u] = 0;
             if (q < total) {
-                const int type = (int)(q / per_type) + 1;
-                long long r = q - (long long)(type - 1) * per_type;
-                // ii: largest with C(ii,3) <= r
-                int ii = (int)cbrt(6.0 * (double)r) + 1;
-                while (ii > 2 && c3(ii) > r) --ii;
-                while (c3(ii + 1) <= r) ++ii;
-                r -= c3(ii);
-                int jj = (int)((1.0 + sqrt(1.0 + 8.0 * (double)r)) * 0.5);
-                while (jj > 1 && (long long)jj * (jj - 1) / 2 > r) --jj;
-                while ((long long)(jj + 1) * jj / 2 <= r) ++jj;
-                const int kk = (int)(r - (long long)jj * (jj - 1) / 2);
                 const double val = tri_value(X, ld, type, ii, jj, kk);
                 vmax = fmax(vmax, val);
                 if (val > thr) {
@@ -121,7 +129,24 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
                     id[u] = ((unsigned)(type - 1) << 30) | ((unsigned)ii << 20) | ((unsigned)jj << 10) | (unsigned)kk;
                     ++nv;
                 }
+                if (++kk == jj) {
+                    kk = 0;
+                    if (++jj == ii) {
+                        jj = 1;
+                        if (++ii == n) { ii = 2; ++type; }
+                    }
+                }
             }
+        }
+        // one barrier when no triple of the chunk is violated (the usual case)
+        {
+            const unsigned long long b = __ballot(nv > 0);
+            if ((threadIdx.x & 63) == 0) s_any[parity][threadIdx.x >> 6] = b != 0ull;
+            __syncthreads();
+            int any = 0;
+#pragma unroll
+            for (int w = 0; w < TT / 64; ++w) any |= s_any[parity][w];
+            if (!any) continue;
         }
         // ordered compaction of the chunk's violated triples
         s_cnt[threadIdx.x] = nv;

@@ -168,6 +168,12 @@ int bqb_ipm_ld_for(int nmax);
 int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
                   int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap);
 
+/* Tuning hook (no counterpart in the reference): force the IPM register
+ * tile (P*100 + R, one of 1604-1607, 3204, 3205) for contexts and batches
+ * created afterwards whose size it covers; 0 restores the size rule.
+ * Returns -1 for an unknown tile. */
+int bqb_set_ipm_tile(int tile);
+
 /* ---- batch: the full node bound (SDPbound) ----------------------------- */
 /* A context holds the device arenas of up to max_nodes B&B nodes of one
  * instance: SP_L is the reference's SP->L (n0 x n0, column-major, built as

@@ -16,6 +16,7 @@ def declare(L):
     L.bqb_dist_selftest.argtypes = [C.c_int, C.c_int, C.c_int, N.dptr, N.dptr, N.iptr]
     L.bqb_dist_selftest.restype = C.c_int
     L.bqb_copy_probe.argtypes = [C.c_size_t, C.c_int, N.dptr]
+    L.bqb_set_ipm_tile.argtypes = [C.c_int]
     L.bqb_ipm_ld_for.argtypes = [C.c_int]
     L.bqb_ipm_ld_for.restype = C.c_int
     _declare_ctx(L)

@@ -3,7 +3,9 @@
 //   bundle_qp_kernel:   zeta = -F - G'gamma, Q = t G'G, then up to 50 rounds of
 //                       { c = zeta - t G'eta; lambda = argmin of the simplex QP;
 //                         eta = max(0, -gamma/t + G lambda);
-//                         dgamma = t (eta - G lambda) }  and gamma_test.
+//                         dgamma = t (eta - G lambda) }  and gamma_test;
+//                       G is read once per round (G lambda and G'eta of the
+//                       next round from the same loads).
 //   bundle_step_kernel: after fct_eval(gamma_test) -- the serious / null step
 //                       test, X = X_bundle lambda, purge and append.
 // The simplex QP (k <= 64 bundle elements) is solved by the reference's
@@ -14,6 +16,7 @@
 
 #include "bqb_batch.h"
 #include "bqb_cutops.h"
+#include "bqb_dist.h"
 #include "bqb_kernels.h"
 
 namespace bqb {
@@ -157,6 +160,124 @@ __device__ int solve_lambda_wave(int k, const double *Q, const double *c, double
     return 0;
 }
 
+// The same algorithm for k <= KR with the factor in registers: lane i holds
+// row i of A = Q + Diag(z./lambda) and of its Cholesky factor; the column of
+// each step is published through LDS, the back substitution reads the
+// transpose from LDS.  Every arithmetic operation is the one of the LDS
+// version above, in the same order, so the results are identical.
+constexpr int KR = 32;
+
+__device__ int solve_lambda_reg(int k, const double *Q, const double *c, double *lam, double *T,
+                                int lane)
+{
+    const bool in = lane < k;
+    double l = in ? 1.0 / k : 0.0;
+    if (in) lam[lane] = l;
+    __builtin_amdgcn_wave_barrier();
+    auto qlam = [&]() {
+        double s = 0.0;
+        if (in)
+            for (int j = 0; j < k; ++j) s += Q[lane + j * k] * lam[j];
+        return s;
+    };
+    double Ql = qlam();
+    const double ci = in ? c[lane] : 0.0;
+    const double tmpv = in ? Ql + ci : 1e300;
+    const double mintmp = wmin(tmpv);
+    double y, z;
+    if (mintmp > 1) { y = 0.0; z = in ? tmpv : 0.0; }
+    else { y = mintmp - 1.0; z = in ? tmpv - y : 0.0; }
+    double mu = wsum(in ? z * l : 0.0) / k * 0.5;
+    double res_p = 1.0 - wsum(in ? l : 0.0);
+    double temp = wsum(in ? l * Ql : 0.0);
+    double D_cost = y - 0.5 * temp;
+    double P_cost = wsum(in ? l * ci : 0.0) + 0.5 * temp;
+    double gap = P_cost - D_cost;
+    int cnt = 0;
+    double Ar[KR], Lt[KR];
+    while (fabs(gap) > 1e-5) {
+        ++cnt;
+        // row `lane` of A = Q with the diagonal -(-Q_ii - z/l)
+        static_for<0, KR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            double v = 0.0;
+            if (j < k && in) v = Q[lane + j * k];
+            if (j == lane) v = -(-v - z / l);
+            Ar[j] = v;
+        });
+        // right-looking Cholesky (wave_chol): column j, then the rank-1
+        // update of the rows below, l <= row
+        bool ok = true;
+        static_for<0, KR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j >= k || !ok) return;
+            const double dj = __shfl(Ar[j], j, 64);
+            if (!(dj > 0.0)) { ok = false; return; }
+            const double ljj = sqrt(dj);
+            if (lane > j && lane < k) Ar[j] = Ar[j] / ljj;
+            if (lane == j) Ar[j] = ljj;
+            T[lane] = Ar[j];
+            __builtin_amdgcn_wave_barrier();
+            const double lij = Ar[j];
+            static_for<j + 1, KR>([&](auto lc) {
+                constexpr int l2 = decltype(lc)::value;
+                if (l2 < k && lane > j && lane < k && l2 <= lane) Ar[l2] -= lij * T[l2];
+            });
+            __builtin_amdgcn_wave_barrier();
+        });
+        if (!ok) return ERR_QP;
+        // the factor's transpose: lane i gets column i (L[j][i], j >= i)
+        static_for<0, KR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            T[lane * (KR + 1) + j] = Ar[j];
+        });
+        __builtin_amdgcn_wave_barrier();
+        static_for<0, KR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            Lt[j] = (in && j < k) ? T[j * (KR + 1) + lane] : 0.0;
+        });
+        __builtin_amdgcn_wave_barrier();
+        // A u = e and A w = r1 (wave_cholsolve on both right-hand sides)
+        double u = 1.0, w = ci - y + Ql;
+        static_for<0, KR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j >= k) return;
+            if (lane == j) { u = u / Ar[j]; w = w / Ar[j]; }
+            const double uj = __shfl(u, j, 64), wj = __shfl(w, j, 64);
+            if (lane > j && lane < k) { u -= Ar[j] * uj; w -= Ar[j] * wj; }
+        });
+        static_for<0, KR>([&](auto jc) {
+            constexpr int j = KR - 1 - decltype(jc)::value;
+            if (j >= k) return;
+            if (lane == j) { u = u / Ar[j]; w = w / Ar[j]; }
+            const double uj = __shfl(u, j, 64), wj = __shfl(w, j, 64);
+            if (lane < j) { u -= Lt[j] * uj; w -= Lt[j] * wj; }
+        });
+        const double ui = in ? u : 0.0, wi = in ? w : 0.0;
+        const double su = wsum(ui), sw = wsum(wi);
+        const double dy = (res_p + sw) / su;
+        const double dl = in ? ui * dy - wi : 0.0;
+        const double dz = in ? 1.0 / l * (mu - l * z - z * dl) : 0.0;
+        double ap = wmax(in ? -dl / l : -1e9);
+        double ad = wmax(in ? -dz / z : -1e9);
+        ap = (ap > 0) ? ((0.99 / ap < 1.0) ? 0.99 / ap : 1.0) : 1.0;
+        ad = (ad > 0) ? ((0.99 / ad < 1.0) ? 0.99 / ad : 1.0) : 1.0;
+        y += ad * dy;
+        if (in) { l = l + ap * dl; z = z + ad * dz; lam[lane] = l; }
+        __builtin_amdgcn_wave_barrier();
+        res_p = 1 - wsum(in ? l : 0.0);
+        Ql = qlam();
+        mu = wsum(in ? l * z : 0.0) / k * 0.4;
+        if (ap + ad > 1.8) mu *= 0.2;
+        temp = wsum(in ? l * Ql : 0.0);
+        D_cost = y - 0.5 * temp;
+        P_cost = wsum(in ? l * ci : 0.0) + 0.5 * temp;
+        gap = P_cost - D_cost;
+        if (cnt > 30) break;
+    }
+    return 0;
+}
+
 __global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
 {
     __shared__ double s_Q[KMAX * KMAX];
@@ -180,11 +301,21 @@ __global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
     const double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
     const double *F = p.F + (size_t)node * d.max_bundle;
     const double *gam = p.gamma + (size_t)node * d.max_m;
-    double *eta = p.eta + (size_t)node * d.max_m;
-    double *dg = p.dgamma + (size_t)node * d.max_m;
     double *gt = p.gtest + (size_t)node * d.max_m;
+    // G[l + j*m] through a buffer resource: byte 8*l [VGPR] + j*8*m [SGPR];
+    // columns past the bundle read as 0
+    const __amdgpu_buffer_rsrc_t Gr =
+        __builtin_amdgcn_make_buffer_rsrc((void *)G, (short)0, 8 * m * k, 0x00020000);
+    const int colb = 8 * m;
+    // per-thread partial sums of G'eta for bundle elements past KR
+    // ((k - KR) * BT <= max_m doubles of this node's scratch)
     double *tmp = p.tmpm + (size_t)node * d.max_m;
     double t = s.t;
+    if (k > KR && (size_t)(k - KR) * BT > (size_t)d.max_m) {
+        if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
+        return;
+    }
+    for (int j = KR; j < k; ++j) tmp[(size_t)(j - KR) * BT + threadIdx.x] = 0.0;
 
     // zeta = -F - G' gamma  (bundle.c:57-63)
     for (int j = wave; j < k; j += BT / 64) {
@@ -203,40 +334,64 @@ __global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
         acc = wsum(acc);
         if (lane == 0) s_Q[i + j * k] = s_Q[j + i * k] = (m > 0) ? t * acc : 0.0;
     }
-    for (int l = threadIdx.x; l < m; l += BT) eta[l] = 0.0;
     if (threadIdx.x == 0) { s_done = 0; s_stat = 0; }
+    // first round: eta = 0, so c = zeta - t G'0
+    for (int j = threadIdx.x; j < k; j += BT) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * 0.0 : s_zeta[j];
     __syncthreads();
 
+    // lambda-eta loop (bundle.c:316-372).  One pass over G per round: each
+    // row of G is loaded once and gives (G lambda)_l, eta_l, dgamma_l and
+    // its share of G'eta for the next round's c.
     double dir_prev = 0.0;
     int cnt = 0;
     while (!s_done) {
         ++cnt;
-        // c = zeta - t G' eta
-        for (int j = wave; j < k; j += BT / 64) {
-            double acc = 0.0;
-            for (int l = lane; l < m; l += 64) acc += G[l + (size_t)j * m] * eta[l];
-            acc = wsum(acc);
-            if (lane == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * acc : s_zeta[j];
-        }
-        __syncthreads();
         if (wave == 0) {
-            const int st = solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
+            const int st = (k <= KR) ? solve_lambda_reg(k, s_Q, s_c, s_lam, s_A, lane)
+                                     : solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
             if (lane == 0 && st) s_stat = st;
         }
         __syncthreads();
         if (s_stat) break;
-        // tmp = G lambda; eta = max(0, -gamma/t + tmp); dgamma = t (eta - tmp)
         double nrm = 0.0;
+        double pe[KR];
+        static_for<0, KR>([&](auto jc) { pe[decltype(jc)::value] = 0.0; });
         for (int l = threadIdx.x; l < m; l += BT) {
+            // row l of G in chunks of 8 columns while the chunk starts inside
+            // the bundle; the second read (for G'eta) hits the cache
+            const int vo = 8 * l;
             double acc = 0.0;
-            for (int j = 0; j < k; ++j) acc += G[l + (size_t)j * m] * s_lam[j];
-            tmp[l] = acc;
+            static_for<0, KR / 8>([&](auto cc) {
+                constexpr int c0 = 8 * decltype(cc)::value;
+                if (c0 >= k) return;
+                double g[8];
+                static_for<0, 8>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    g[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, (c0 + j) * colb, 0));
+                });
+                static_for<0, 8>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    if (c0 + j < k) acc += g[j] * s_lam[c0 + j];
+                });
+            });
+            for (int j = KR; j < k; ++j)
+                acc += __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, j * colb, 0)) * s_lam[j];
             double e = -gam[l] / t + acc;
             e = (e > 0) ? e : 0;
-            eta[l] = e;
             const double dgl = t * (e - acc);
-            dg[l] = dgl;
+            gt[l] = gam[l] + dgl;
             nrm += dgl * dgl;
+            static_for<0, KR / 8>([&](auto cc) {
+                constexpr int c0 = 8 * decltype(cc)::value;
+                if (c0 >= k) return;
+                static_for<0, 8>([&](auto jc) {
+                    constexpr int j = c0 + decltype(jc)::value;
+                    pe[j] += __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, j * colb, 0)) * e;
+                });
+            });
+            for (int j = KR; j < k; ++j)
+                tmp[(size_t)(j - KR) * BT + threadIdx.x] +=
+                    __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, j * colb, 0)) * e;
         }
         nrm = bsum(nrm, s_red);
         const double dir = sqrt(nrm);
@@ -250,12 +405,25 @@ __global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
         dir_prev = dir;
         __syncthreads();
         t = s_t;
+        if (s_done) break;
+        // c = zeta - t G'eta for the next round
+        static_for<0, KR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j >= k) return;
+            const double v = bsum(pe[j], s_red);
+            if (threadIdx.x == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
+        });
+        for (int j = KR; j < k; ++j) {
+            const double v = bsum(tmp[(size_t)(j - KR) * BT + threadIdx.x], s_red);
+            if (threadIdx.x == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
+        }
+        for (int j = KR; j < k; ++j) tmp[(size_t)(j - KR) * BT + threadIdx.x] = 0.0;
+        __syncthreads();
     }
     if (s_stat) {
         if (threadIdx.x == 0) { sp->status = s_stat; sp->active = 0; p.mask[node] = 0; }
         return;
     }
-    for (int l = threadIdx.x; l < m; l += BT) gt[l] = gam[l] + dg[l];
     double *lam = p.lambda + (size_t)node * d.max_bundle;
     for (int j = threadIdx.x; j < k; j += BT) lam[j] = s_lam[j];
     if (threadIdx.x == 0) sp->t = t;

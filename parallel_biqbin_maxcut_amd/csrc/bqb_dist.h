@@ -174,8 +174,9 @@ __device__ __forceinline__ void row_sums(const double (&part)[R], double *red, d
 
 // ---------------------------------------------------------------- sweep inverse
 // In-place inverse of an SPD matrix by the symmetric sweep operator
-// (Beaton / Goodnight): after sweeping every pivot the matrix holds -inv(A),
-// which is negated at the end.  Returns false (uniformly) if a pivot is not
+// (Beaton / Goodnight): pivot k maps a_ij -> a_ij - a_ik (a_kj / a_kk),
+// a_ik -> a_ik / a_kk, a_kk -> -1 / a_kk; after every pivot the matrix holds
+// -inv(A), which is negated at the end.  Returns false (uniformly) if a pivot is not
 // positive, i.e. A is not positive definite -- the same test as dpotrf.
 template <int P, int R>
 __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *colbuf, const Thr &t)
@@ -195,28 +196,29 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
                 for (int a = 0; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
             }
             __syncthreads();
+            // every LDS read of the step is issued before the pivot test
             const double d = cb[k];
-            if (!(d > 0.0)) { ok = false; break; }
-            const double dinv = 1.0 / d;
-            const double s = sqrt(dinv);
-            double cI[R], sJ[R];
+            double cI[R], cJ[R];
 #pragma unroll
             for (int a = 0; a < R; ++a) cI[a] = cb[t.tr + P * a];
 #pragma unroll
-            for (int b = 0; b < R; ++b) sJ[b] = cb[t.tc + P * b] * s;
+            for (int b = 0; b < R; ++b) cJ[b] = cb[t.tc + P * b];
+            if (!(d > 0.0)) { ok = false; break; }
+            const double dinv = 1.0 / d;
+            double tJ[R];
 #pragma unroll
-            for (int a = 0; a < R; ++a) {
-                const double sI = cI[a] * s;
+            for (int b = 0; b < R; ++b) tJ[b] = cJ[b] * dinv;
 #pragma unroll
-                for (int b = 0; b < R; ++b) A[a][b] = fma(-sI, sJ[b], A[a][b]);
-            }
+            for (int a = 0; a < R; ++a)
+#pragma unroll
+                for (int b = 0; b < R; ++b) A[a][b] = fma(-cI[a], tJ[b], A[a][b]);
             if (t.tc == kt) {
 #pragma unroll
                 for (int a = 0; a < R; ++a) A[a][kb] = cI[a] * dinv;
             }
             if (t.tr == kt) {
 #pragma unroll
-                for (int b = 0; b < R; ++b) A[kb][b] = cb[t.tc + P * b] * dinv;
+                for (int b = 0; b < R; ++b) A[kb][b] = tJ[b];
             }
             if (t.tr == kt && t.tc == kt) A[kb][kb] = -dinv;
             buf ^= 1;
@@ -230,8 +232,9 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
     return true;
 }
 
-// Positive-definiteness test by right-looking Cholesky (destroys A).  Same
-// decision rule as dpotrf: fails at the first pivot that is not > 0.
+// Positive-definiteness test by right-looking Cholesky (destroys A): the
+// trailing matrix is updated with c_i (c_j / d).  Same decision rule as
+// dpotrf: fails at the first pivot that is not > 0.
 template <int P, int R>
 __device__ __forceinline__ bool chol_test(double (&A)[R][R], int n, double *colbuf, const Thr &t)
 {
@@ -250,18 +253,22 @@ __device__ __forceinline__ bool chol_test(double (&A)[R][R], int n, double *colb
                 for (int a = kb; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
             }
             __syncthreads();
+            // every LDS read of the step is issued before the pivot test
             const double d = cb[k];
+            double cI[R], cJ[R];
+#pragma unroll
+            for (int a = kb; a < R; ++a) cI[a] = cb[t.tr + P * a];
+#pragma unroll
+            for (int b = kb; b < R; ++b) cJ[b] = cb[t.tc + P * b];
             if (!(d > 0.0)) { ok = false; break; }
-            const double s = 1.0 / sqrt(d);
-            double sJ[R];
+            const double dinv = 1.0 / d;
+            double tJ[R];
 #pragma unroll
-            for (int b = kb; b < R; ++b) sJ[b] = cb[t.tc + P * b] * s;
+            for (int b = kb; b < R; ++b) tJ[b] = cJ[b] * dinv;
 #pragma unroll
-            for (int a = kb; a < R; ++a) {
-                const double sI = cb[t.tr + P * a] * s;
+            for (int a = kb; a < R; ++a)
 #pragma unroll
-                for (int b = kb; b < R; ++b) A[a][b] = fma(-sI, sJ[b], A[a][b]);
-            }
+                for (int b = kb; b < R; ++b) A[a][b] = fma(-cI[a], tJ[b], A[a][b]);
             buf ^= 1;
         }
     });
@@ -288,17 +295,23 @@ __device__ __forceinline__ bool chol_lower(double (&A)[R][R], int n, double *col
                 for (int a = kb; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
             }
             __syncthreads();
+            // every LDS read of the step is issued before the pivot test
             const double d = cb[k];
+            double cI[R], cJ[R];
+#pragma unroll
+            for (int a = kb; a < R; ++a) cI[a] = cb[t.tr + P * a];
+#pragma unroll
+            for (int b = kb; b < R; ++b) cJ[b] = cb[t.tc + P * b];
             if (!(d > 0.0)) { ok = false; break; }
             const double sq = sqrt(d);
             const double s = 1.0 / sq;
             double sJ[R];
 #pragma unroll
-            for (int b = kb; b < R; ++b) sJ[b] = cb[t.tc + P * b] * s;
+            for (int b = kb; b < R; ++b) sJ[b] = cJ[b] * s;
 #pragma unroll
             for (int a = kb; a < R; ++a) {
                 const int I = t.tr + P * a;
-                const double sI = cb[I] * s;
+                const double sI = cI[a] * s;
 #pragma unroll
                 for (int b = kb; b < R; ++b) {
                     const int J = t.tc + P * b;
@@ -309,7 +322,7 @@ __device__ __forceinline__ bool chol_lower(double (&A)[R][R], int n, double *col
 #pragma unroll
                 for (int a = kb; a < R; ++a) {
                     const int I = t.tr + P * a;
-                    if (I >= k) A[a][kb] = (I == k) ? sq : cb[I] * s;
+                    if (I >= k) A[a][kb] = (I == k) ? sq : cI[a] * s;
                 }
             }
             buf ^= 1;

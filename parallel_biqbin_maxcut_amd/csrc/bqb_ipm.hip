@@ -32,6 +32,11 @@
 namespace bqb {
 
 constexpr int IPM_MAX_ITERS = 500;
+
+// Keeps the loads of one register column from being hoisted above the
+// previous column: caps the registers held by loads in flight, so the kernel
+// fits two waves per SIMD (two subproblems per CU).
+#define BQB_COLUMN_FENCE() __builtin_amdgcn_sched_barrier(0)
 constexpr int LS_MAX_TRIALS = 500;
 constexpr int KB = 8;   // GEMM panel depth
 
@@ -77,21 +82,23 @@ __device__ __forceinline__ void gemm_sym(double (&C)[R][R], const double *Ag, in
     __syncthreads();
 }
 
+// LDS of one solve (sized by the kernel for its largest tile).
+struct IpmLds {
+    double *col, *scr, *y, *zd, *dy1, *dy2, *v, *dg, *red;
+};
+
+// The solve of one subproblem with an R x R register tile per thread (the
+// smallest tile that covers its n; the slabs keep the kernel's leading
+// dimension).
 template <int P, int R>
-__global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
+__device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int node, const int n,
+                                          const IpmLds &S)
 {
     constexpr int NMAX = P * R;
-    constexpr int NW = P * P / 64;
-    using L_ = Lds<P, R, KB>;
-    __shared__ double s_col[L_::COL];
-    __shared__ double s_scr[L_::UNION];
-    __shared__ double s_y[NMAX], s_zd[NMAX], s_dy1[NMAX], s_dy2[NMAX], s_v[NMAX], s_dg[NMAX];
-    __shared__ double s_red[NW];
-
-    const int node = blockIdx.x;
-    if (args.active && !args.active[node]) return;
+    double *const s_col = S.col, *const s_scr = S.scr, *const s_y = S.y, *const s_zd = S.zd;
+    double *const s_dy1 = S.dy1, *const s_dy2 = S.dy2, *const s_v = S.v, *const s_dg = S.dg;
+    double *const s_red = S.red;
     const Thr t = thr_id<P>();
-    const int n = args.n[node];
     const int ld = args.ld;
     const size_t nn = (size_t)ld * ld;
     const double *L = args.L + (size_t)node * nn;
@@ -102,11 +109,6 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
     const Slab sL = make_slab<P>(L, ld, t), sX = make_slab<P>(X, ld, t);
     const Slab sZi = make_slab<P>(Wzi, ld, t), sMi = make_slab<P>(Wmi, ld, t);
     const Slab sDx = make_slab<P>(Wdx, ld, t);
-
-    if (n < 1 || n > NMAX || ld < NMAX) {
-        if (t.tid == 0) { args.status[node] = ERR_SIZE; args.iters[node] = 0; }
-        return;
-    }
 
     double A[R][R];
 
@@ -261,10 +263,12 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
             return (-s_dy2[k] * X[idx]) + (-s_dy1[k] * Wdx[idx]);
         });
 #pragma unroll
-        for (int b = 0; b < R; ++b)
+        for (int b = 0; b < R; ++b) {
 #pragma unroll
             for (int a = 0; a < R; ++a)
                 A[a][b] = (A[a][b] + mu * sld<P>(sZi, a, b)) + sld<P>(sDx, a, b);
+            BQB_COLUMN_FENCE();
+        }
         symmetrize<P, R>(A, s_scr, t);
         dstore<P, R>(A, sDx);                         // dX
         for (int I = t.tid; I < NMAX; I += P * P) s_v[I] = s_dy2[I] + s_dy1[I];   // dy
@@ -277,10 +281,12 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
             bool ok = false;
             while (tr_ < LS_MAX_TRIALS) {
 #pragma unroll
-                for (int b = 0; b < R; ++b)
+                for (int b = 0; b < R; ++b) {
 #pragma unroll
                     for (int a = 0; a < R; ++a)
                         A[a][b] = sld<P>(sX, a, b) + alpha_p * sld<P>(sDx, a, b);
+                    BQB_COLUMN_FENCE();
+                }
                 ++tr_;
                 if (chol_test<P, R>(A, n, s_col, t)) { ok = true; break; }
                 alpha_p *= 0.8;
@@ -318,10 +324,12 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
 
         // ---- update -------------------------------------------------------
 #pragma unroll
-        for (int b = 0; b < R; ++b)
+        for (int b = 0; b < R; ++b) {
 #pragma unroll
             for (int a = 0; a < R; ++a)
                 A[a][b] = sld<P>(sX, a, b) + alpha_p * sld<P>(sDx, a, b);
+            BQB_COLUMN_FENCE();
+        }
         dstore<P, R>(A, sX);
         for (int I = t.tid; I < n; I += P * P) {
             s_y[I] = s_y[I] + alpha_d * s_v[I];
@@ -331,7 +339,7 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
         {
             double pz = 0.0, pl = 0.0;
 #pragma unroll
-            for (int b = 0; b < R; ++b)
+            for (int b = 0; b < R; ++b) {
 #pragma unroll
                 for (int a = 0; a < R; ++a) {
                     const int I = t.tr + P * a, J = t.tc + P * b;
@@ -340,6 +348,8 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
                     pz += z * A[a][b];
                     pl += l * A[a][b];
                 }
+                BQB_COLUMN_FENCE();
+            }
             double py = 0.0;
             for (int I = t.tid; I < n; I += P * P) py += s_y[I];
             mu = wg_sum<P>(pz, s_red, t) / (2.0 * n);
@@ -363,6 +373,35 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
         for (int I = t.tid; I < n; I += P * P) args.y_out[(size_t)node * args.ystride + I] = s_y[I];
 }
 
+// One workgroup per subproblem.  The register tile is chosen per
+// subproblem: ceil(n / P) blocks, so a small node does not pay for the
+// padding of the largest one.
+template <int P, int R>
+__global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
+{
+    constexpr int NMAX = P * R;
+    constexpr int NW = P * P / 64;
+    using L_ = Lds<P, R, KB>;
+    __shared__ double s_col[L_::COL];
+    __shared__ double s_scr[L_::UNION];
+    __shared__ double s_y[NMAX], s_zd[NMAX], s_dy1[NMAX], s_dy2[NMAX], s_v[NMAX], s_dg[NMAX];
+    __shared__ double s_red[NW];
+
+    const int node = blockIdx.x;
+    if (args.active && !args.active[node]) return;
+    const int n = args.n[node];
+    if (n < 1 || n > NMAX || args.ld < NMAX) {
+        if (threadIdx.x == 0) { args.status[node] = ERR_SIZE; args.iters[node] = 0; }
+        return;
+    }
+    const IpmLds S = {s_col, s_scr, s_y, s_zd, s_dy1, s_dy2, s_v, s_dg, s_red};
+    const int re = (n + P - 1) / P;
+    static_for<1, R + 1>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if (re == r) ipm_solve<P, r>(args, node, n, S);
+    });
+}
+
 template <int P, int R>
 static hipError_t launch_ipm(const IpmBatchArgs &a, int nodes, hipStream_t s)
 {
@@ -375,8 +414,24 @@ static hipError_t launch_ipm(const IpmBatchArgs &a, int nodes, hipStream_t s)
 using namespace bqb;
 
 // Picks the register-tile shape for the largest n in the batch.
+static int g_ipm_tile = 0;   // 0: by size; else a forced tile (tuning)
+
+// Forces the IPM register tile (P*100 + R: 1604 ... 3205) for later
+// contexts and batches whose size it covers; 0 restores the size rule.
+extern "C" int bqb_set_ipm_tile(int tile)
+{
+    switch (tile) {
+    case 0: case 1604: case 1605: case 1606: case 1607: case 3204: case 3205:
+        g_ipm_tile = tile;
+        return 0;
+    default:
+        return -1;
+    }
+}
+
 extern "C" int bqb_ipm_tile_for(int nmax)
 {
+    if (g_ipm_tile && (g_ipm_tile / 100) * (g_ipm_tile % 100) >= nmax) return g_ipm_tile;
     if (nmax <= 64) return 1604;
     if (nmax <= 80) return 1605;
     if (nmax <= 96) return 1606;

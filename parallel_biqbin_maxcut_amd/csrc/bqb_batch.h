@@ -68,6 +68,7 @@ typedef struct BatchDims {
     int max_contrib;    /* op_Bt gather capacity */
     int npairs;         /* ld*(ld-1)/2 */
     int sa_chains5, sa_chains7;   /* SA chains per pentagonal / heptagonal round */
+    int tmp_ld;         /* per-node length of tmpm */
 } BatchDims;
 
 /* Device pointers of a batch (all arrays are per node at the stated stride). */
@@ -85,7 +86,8 @@ typedef struct BatchPtrs {
     double *Xb;                     /* [nodes*max_bundle*ld*ld] bundle matrices */
     double *G;                      /* [nodes*max_bundle*max_m], column i at i*m (ld = m) */
     double *F, *lambda, *zeta;      /* [nodes*max_bundle] */
-    double *gamma, *dgamma, *gtest, *eta, *g, *tmpm;   /* [nodes*max_m] */
+    double *gamma, *dgamma, *gtest, *eta, *g;   /* [nodes*max_m] */
+    double *tmpm;                   /* [nodes*tmp_ld] bundle QP partial sums */
     TriCut *tri;                    /* [nodes*max_tri] */
     KgonCut *pent, *hept;           /* [nodes*max_pent], [nodes*max_hept] */
     double *tri_y, *pent_y, *hept_y;/* dual multipliers stored with the cuts */
@@ -108,6 +110,11 @@ typedef struct BatchPtrs {
     int *ld_xf, *ld_sx;             /* [nodes*(n0-1)] */
     int *ld_of_slot;                /* [nodes]: index into the staging, or -1 */
 } BatchPtrs;
+
+/* bundle QP: threads per node, and bundle elements whose G'eta partial sums
+   stay in registers (the others use tmpm, BQB_QP_THREADS doubles each) */
+#define BQB_QP_THREADS 256
+#define BQB_QP_REG_PARTIALS 16
 
 #define BQB_SA_CHAINS 256           /* >= max(Pent_Trials*3, Hepta_Trials*4) supported */
 #define BQB_TRI_LIST_MAX 4096       /* largest TriIneq (triangle list capacity) supported */

@@ -18,27 +18,16 @@
 #include "bqb_cutops.h"
 #include "bqb_dist.h"
 #include "bqb_kernels.h"
+#include "bqb_wave.h"
 
 namespace bqb {
 
-constexpr int BT = 256;
+constexpr int BT = BQB_QP_THREADS;
 constexpr int KMAX = 64;
 
-__device__ __forceinline__ double wsum(double v)
-{
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wmax(double v)
-{
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wmin(double v)
-{
-    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
+__device__ __forceinline__ double wsum(double v) { return wave_sum64(v); }
+__device__ __forceinline__ double wmax(double v) { return wave_max64(v); }
+__device__ __forceinline__ double wmin(double v) { return wave_min64(v); }
 
 __device__ __forceinline__ double bsum(double v, double *red)
 {
@@ -160,13 +149,19 @@ __device__ int solve_lambda_wave(int k, const double *Q, const double *c, double
     return 0;
 }
 
-// The same algorithm for k <= KR with the factor in registers: lane i holds
-// row i of A = Q + Diag(z./lambda) and of its Cholesky factor; the column of
-// each step is published through LDS, the back substitution reads the
-// transpose from LDS.  Every arithmetic operation is the one of the LDS
-// version above, in the same order, so the results are identical.
+// The same algorithm for k <= K (K = 8, 16, 32) with the factor in
+// registers: lane i holds row i of A = Q + Diag(z./lambda) and of its
+// Cholesky factor.  Pivots are read with v_readlane, each step's column is
+// stored to LDS (the back substitution reads the transpose from there).
+// The rank-1 updates run unmasked: lanes at or above k and entries above
+// the diagonal collect values that are never read.  Divisions by a pivot
+// are multiplications by its reciprocal (one divide per column), so results
+// agree with the LDS version to rounding, not bit for bit.
 constexpr int KR = 32;
+// bundle elements whose G'eta partial sums stay in registers in the row pass
+constexpr int PE = BQB_QP_REG_PARTIALS;
 
+template <int K>
 __device__ int solve_lambda_reg(int k, const double *Q, const double *c, double *lam, double *T,
                                 int lane)
 {
@@ -183,102 +178,101 @@ __device__ int solve_lambda_reg(int k, const double *Q, const double *c, double 
     double Ql = qlam();
     const double ci = in ? c[lane] : 0.0;
     const double tmpv = in ? Ql + ci : 1e300;
-    const double mintmp = wmin(tmpv);
+    const double mintmp = wave_min32(tmpv);
     double y, z;
     if (mintmp > 1) { y = 0.0; z = in ? tmpv : 0.0; }
     else { y = mintmp - 1.0; z = in ? tmpv - y : 0.0; }
-    double mu = wsum(in ? z * l : 0.0) / k * 0.5;
-    double res_p = 1.0 - wsum(in ? l : 0.0);
-    double temp = wsum(in ? l * Ql : 0.0);
+    double mu = wave_sum32(in ? z * l : 0.0) / k * 0.5;
+    double res_p = 1.0 - wave_sum32(in ? l : 0.0);
+    double temp = wave_sum32(in ? l * Ql : 0.0);
     double D_cost = y - 0.5 * temp;
-    double P_cost = wsum(in ? l * ci : 0.0) + 0.5 * temp;
+    double P_cost = wave_sum32(in ? l * ci : 0.0) + 0.5 * temp;
     double gap = P_cost - D_cost;
     int cnt = 0;
-    double Ar[KR], Lt[KR];
+    const int qrow = in ? lane : 0;
+    double Ar[K], Lt[K];
     while (fabs(gap) > 1e-5) {
         ++cnt;
         // row `lane` of A = Q with the diagonal -(-Q_ii - z/l)
-        static_for<0, KR>([&](auto jc) {
+        const double zl = in ? z / l : 0.0;
+        static_for<0, K>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            double v = 0.0;
-            if (j < k && in) v = Q[lane + j * k];
-            if (j == lane) v = -(-v - z / l);
-            Ar[j] = v;
+            const double q = Q[qrow + (j < k ? j : 0) * k];
+            Ar[j] = (j == lane) ? -(-q - zl) : q;
         });
-        // right-looking Cholesky (wave_chol): column j, then the rank-1
-        // update of the rows below, l <= row
+        // right-looking Cholesky (wave_chol).  Column j goes to LDS at
+        // T[j*(K+1) + i]; lane j keeps 1/L_jj in rd.
         bool ok = true;
-        static_for<0, KR>([&](auto jc) {
+        double rd = 0.0;
+        static_for<0, K>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
             if (j >= k || !ok) return;
-            const double dj = __shfl(Ar[j], j, 64);
+            const double dj = lane_value(Ar[j], j);
             if (!(dj > 0.0)) { ok = false; return; }
             const double ljj = sqrt(dj);
-            if (lane > j && lane < k) Ar[j] = Ar[j] / ljj;
-            if (lane == j) Ar[j] = ljj;
-            T[lane] = Ar[j];
+            const double rj = 1.0 / ljj;
+            const double lij = (lane == j) ? ljj : Ar[j] * rj;
+            Ar[j] = lij;
+            if (lane == j) rd = rj;
+            T[j * (K + 1) + lane] = lij;
             __builtin_amdgcn_wave_barrier();
-            const double lij = Ar[j];
-            static_for<j + 1, KR>([&](auto lc) {
+            static_for<j + 1, K>([&](auto lc) {
                 constexpr int l2 = decltype(lc)::value;
-                if (l2 < k && lane > j && lane < k && l2 <= lane) Ar[l2] -= lij * T[l2];
+                Ar[l2] -= lij * T[j * (K + 1) + l2];
             });
-            __builtin_amdgcn_wave_barrier();
         });
         if (!ok) return ERR_QP;
-        // the factor's transpose: lane i gets column i (L[j][i], j >= i)
-        static_for<0, KR>([&](auto jc) {
+        // the factor's transpose: lane i gets row i of L' (L[j][i], j >= i)
+        static_for<0, K>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            T[lane * (KR + 1) + j] = Ar[j];
-        });
-        __builtin_amdgcn_wave_barrier();
-        static_for<0, KR>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            Lt[j] = (in && j < k) ? T[j * (KR + 1) + lane] : 0.0;
+            Lt[j] = T[lane * (K + 1) + j];
         });
         __builtin_amdgcn_wave_barrier();
         // A u = e and A w = r1 (wave_cholsolve on both right-hand sides)
         double u = 1.0, w = ci - y + Ql;
-        static_for<0, KR>([&](auto jc) {
+        static_for<0, K>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
             if (j >= k) return;
-            if (lane == j) { u = u / Ar[j]; w = w / Ar[j]; }
-            const double uj = __shfl(u, j, 64), wj = __shfl(w, j, 64);
-            if (lane > j && lane < k) { u -= Ar[j] * uj; w -= Ar[j] * wj; }
+            const double rjj = lane_value(rd, j);
+            const double uj = lane_value(u, j) * rjj, wj = lane_value(w, j) * rjj;
+            const bool below = lane > j && in;
+            u = (lane == j) ? uj : (below ? u - Ar[j] * uj : u);
+            w = (lane == j) ? wj : (below ? w - Ar[j] * wj : w);
         });
-        static_for<0, KR>([&](auto jc) {
-            constexpr int j = KR - 1 - decltype(jc)::value;
+        static_for<0, K>([&](auto jc) {
+            constexpr int j = K - 1 - decltype(jc)::value;
             if (j >= k) return;
-            if (lane == j) { u = u / Ar[j]; w = w / Ar[j]; }
-            const double uj = __shfl(u, j, 64), wj = __shfl(w, j, 64);
-            if (lane < j) { u -= Lt[j] * uj; w -= Lt[j] * wj; }
+            const double rjj = lane_value(rd, j);
+            const double uj = lane_value(u, j) * rjj, wj = lane_value(w, j) * rjj;
+            u = (lane == j) ? uj : (lane < j ? u - Lt[j] * uj : u);
+            w = (lane == j) ? wj : (lane < j ? w - Lt[j] * wj : w);
         });
         const double ui = in ? u : 0.0, wi = in ? w : 0.0;
-        const double su = wsum(ui), sw = wsum(wi);
+        const double su = wave_sum32(ui), sw = wave_sum32(wi);
         const double dy = (res_p + sw) / su;
         const double dl = in ? ui * dy - wi : 0.0;
         const double dz = in ? 1.0 / l * (mu - l * z - z * dl) : 0.0;
-        double ap = wmax(in ? -dl / l : -1e9);
-        double ad = wmax(in ? -dz / z : -1e9);
+        double ap = wave_max32(in ? -dl / l : -1e9);
+        double ad = wave_max32(in ? -dz / z : -1e9);
         ap = (ap > 0) ? ((0.99 / ap < 1.0) ? 0.99 / ap : 1.0) : 1.0;
         ad = (ad > 0) ? ((0.99 / ad < 1.0) ? 0.99 / ad : 1.0) : 1.0;
         y += ad * dy;
         if (in) { l = l + ap * dl; z = z + ad * dz; lam[lane] = l; }
         __builtin_amdgcn_wave_barrier();
-        res_p = 1 - wsum(in ? l : 0.0);
+        res_p = 1 - wave_sum32(in ? l : 0.0);
         Ql = qlam();
-        mu = wsum(in ? l * z : 0.0) / k * 0.4;
+        mu = wave_sum32(in ? l * z : 0.0) / k * 0.4;
         if (ap + ad > 1.8) mu *= 0.2;
-        temp = wsum(in ? l * Ql : 0.0);
+        temp = wave_sum32(in ? l * Ql : 0.0);
         D_cost = y - 0.5 * temp;
-        P_cost = wsum(in ? l * ci : 0.0) + 0.5 * temp;
+        P_cost = wave_sum32(in ? l * ci : 0.0) + 0.5 * temp;
         gap = P_cost - D_cost;
         if (cnt > 30) break;
     }
     return 0;
 }
 
-__global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
+__global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
 {
     __shared__ double s_Q[KMAX * KMAX];
     __shared__ double s_A[KMAX * KMAX];
@@ -307,32 +301,72 @@ __global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
     const __amdgpu_buffer_rsrc_t Gr =
         __builtin_amdgcn_make_buffer_rsrc((void *)G, (short)0, 8 * m * k, 0x00020000);
     const int colb = 8 * m;
-    // per-thread partial sums of G'eta for bundle elements past KR
-    // ((k - KR) * BT <= max_m doubles of this node's scratch)
-    double *tmp = p.tmpm + (size_t)node * d.max_m;
+    // per-thread partial sums of G'eta for bundle elements past PE
+    // ((k - PE) * BT <= tmp_ld doubles of this node's scratch)
+    double *tmp = p.tmpm + (size_t)node * d.tmp_ld;
     double t = s.t;
-    if (k > KR && (size_t)(k - KR) * BT > (size_t)d.max_m) {
+    if (k > PE && (size_t)(k - PE) * BT > (size_t)d.tmp_ld) {
         if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
         return;
     }
-    for (int j = KR; j < k; ++j) tmp[(size_t)(j - KR) * BT + threadIdx.x] = 0.0;
+    for (int j = PE; j < k; ++j) tmp[(size_t)(j - PE) * BT + threadIdx.x] = 0.0;
 
-    // zeta = -F - G' gamma  (bundle.c:57-63)
-    for (int j = wave; j < k; j += BT / 64) {
-        double acc = 0.0;
-        for (int l = lane; l < m; l += 64) acc += G[l + (size_t)j * m] * gam[l];
-        acc = wsum(acc);
-        // with no cuts dgemv returns early and leaves zeta = F
-        if (lane == 0) s_zeta[j] = (m > 0) ? -F[j] + (-1.0) * acc : F[j];
-    }
-    // Q = t G'G  (bundle.c:309-315)
-    for (int q = wave; q < k * k; q += BT / 64) {
-        const int i = q % k, j = q / k;
-        if (i < j) continue;
-        double acc = 0.0;
-        for (int l = lane; l < m; l += 64) acc += G[l + (size_t)i * m] * G[l + (size_t)j * m];
-        acc = wsum(acc);
-        if (lane == 0) s_Q[i + j * k] = s_Q[j + i * k] = (m > 0) ? t * acc : 0.0;
+    // zeta = -F - G' gamma (bundle.c:57-63) and Q = t G'G (bundle.c:309-315)
+    // in one pass over the rows of [G gamma], staged through LDS (s_A) in
+    // tiles of R rows.  Each thread owns up to GP of the k(k+1)/2 + k dot
+    // products and sums its rows in order l = 0..m-1.
+    {
+        const int kc = k + 1;                       // column k holds gamma
+        const int RS = min(256 + 1, (KMAX * KMAX) / kc);   // column stride (odd: bank spread)
+        const int R = RS - 1;
+        const int npq = k * (k + 1) / 2;
+        const int np = npq + k;
+        constexpr int GP = (KMAX * (KMAX + 1) / 2 + KMAX + BT - 1) / BT;
+        double acc[GP];
+        int pa[GP], pb[GP];
+        static_for<0, GP>([&](auto sc) {
+            constexpr int q = decltype(sc)::value;
+            const int pp = threadIdx.x + q * BT;
+            acc[q] = 0.0;
+            int i = 0, j = 0;
+            if (pp < npq) {                 // column-major lower triangle, i >= j
+                int r = pp;
+                while (r >= k - j) { r -= k - j; ++j; }
+                i = j + r;
+            } else if (pp < np) {
+                i = pp - npq;
+                j = k;
+            }
+            pa[q] = i * RS;
+            pb[q] = j * RS;
+        });
+        for (int l0 = 0; l0 < m; l0 += R) {
+            const int rows = min(R, m - l0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < rows * kc; e += BT) {
+                const int c = e / rows, l = e - c * rows;
+                s_A[c * RS + l] = (c < k) ? G[l0 + l + (size_t)c * m] : gam[l0 + l];
+            }
+            __syncthreads();
+            static_for<0, GP>([&](auto sc) {
+                constexpr int q = decltype(sc)::value;
+                if (threadIdx.x + q * BT >= np) return;
+                const double *x = s_A + pa[q], *y = s_A + pb[q];
+                double v = acc[q];
+                for (int l = 0; l < rows; ++l) v += x[l] * y[l];
+                acc[q] = v;
+            });
+        }
+        static_for<0, GP>([&](auto sc) {
+            constexpr int q = decltype(sc)::value;
+            const int pp = threadIdx.x + q * BT;
+            if (pp >= np) return;
+            const int i = pa[q] / RS, j = pb[q] / RS;
+            if (pp < npq) s_Q[i + j * k] = s_Q[j + i * k] = (m > 0) ? t * acc[q] : 0.0;
+            // with no cuts dgemv returns early and leaves zeta = F
+            else s_zeta[i] = (m > 0) ? -F[i] + (-1.0) * acc[q] : F[i];
+        });
+        __syncthreads();
     }
     if (threadIdx.x == 0) { s_done = 0; s_stat = 0; }
     // first round: eta = 0, so c = zeta - t G'0
@@ -347,50 +381,53 @@ __global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
     while (!s_done) {
         ++cnt;
         if (wave == 0) {
-            const int st = (k <= KR) ? solve_lambda_reg(k, s_Q, s_c, s_lam, s_A, lane)
-                                     : solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
+            const int st = (k <= 8)    ? solve_lambda_reg<8>(k, s_Q, s_c, s_lam, s_A, lane)
+                           : (k <= 16) ? solve_lambda_reg<16>(k, s_Q, s_c, s_lam, s_A, lane)
+                           : (k <= KR) ? solve_lambda_reg<KR>(k, s_Q, s_c, s_lam, s_A, lane)
+                                       : solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
             if (lane == 0 && st) s_stat = st;
         }
         __syncthreads();
         if (s_stat) break;
         double nrm = 0.0;
-        double pe[KR];
-        static_for<0, KR>([&](auto jc) { pe[decltype(jc)::value] = 0.0; });
+        double pe[PE];
+        static_for<0, PE>([&](auto jc) { pe[decltype(jc)::value] = 0.0; });
         for (int l = threadIdx.x; l < m; l += BT) {
-            // row l of G in chunks of 8 columns while the chunk starts inside
-            // the bundle; the second read (for G'eta) hits the cache
+            // row l of G: the first PE columns are read once, in chunks of 8
+            // while the chunk starts inside the bundle, and kept for G'eta
             const int vo = 8 * l;
+            double g[PE];
             double acc = 0.0;
-            static_for<0, KR / 8>([&](auto cc) {
+            static_for<0, PE / 8>([&](auto cc) {
                 constexpr int c0 = 8 * decltype(cc)::value;
                 if (c0 >= k) return;
-                double g[8];
                 static_for<0, 8>([&](auto jc) {
-                    constexpr int j = decltype(jc)::value;
-                    g[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, (c0 + j) * colb, 0));
+                    constexpr int j = c0 + decltype(jc)::value;
+                    g[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, j * colb, 0));
                 });
                 static_for<0, 8>([&](auto jc) {
-                    constexpr int j = decltype(jc)::value;
-                    if (c0 + j < k) acc += g[j] * s_lam[c0 + j];
+                    constexpr int j = c0 + decltype(jc)::value;
+                    if (j < k) acc += g[j] * s_lam[j];
                 });
             });
-            for (int j = KR; j < k; ++j)
+            for (int j = PE; j < k; ++j)
                 acc += __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, j * colb, 0)) * s_lam[j];
             double e = -gam[l] / t + acc;
             e = (e > 0) ? e : 0;
             const double dgl = t * (e - acc);
             gt[l] = gam[l] + dgl;
             nrm += dgl * dgl;
-            static_for<0, KR / 8>([&](auto cc) {
+            static_for<0, PE / 8>([&](auto cc) {
                 constexpr int c0 = 8 * decltype(cc)::value;
                 if (c0 >= k) return;
                 static_for<0, 8>([&](auto jc) {
                     constexpr int j = c0 + decltype(jc)::value;
-                    pe[j] += __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, j * colb, 0)) * e;
+                    pe[j] += g[j] * e;
                 });
             });
-            for (int j = KR; j < k; ++j)
-                tmp[(size_t)(j - KR) * BT + threadIdx.x] +=
+            // columns past PE are read again (from the cache)
+            for (int j = PE; j < k; ++j)
+                tmp[(size_t)(j - PE) * BT + threadIdx.x] +=
                     __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(Gr, vo, j * colb, 0)) * e;
         }
         nrm = bsum(nrm, s_red);
@@ -407,17 +444,17 @@ __global__ __launch_bounds__(BT) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
         t = s_t;
         if (s_done) break;
         // c = zeta - t G'eta for the next round
-        static_for<0, KR>([&](auto jc) {
+        static_for<0, PE>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
             if (j >= k) return;
             const double v = bsum(pe[j], s_red);
             if (threadIdx.x == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
         });
-        for (int j = KR; j < k; ++j) {
-            const double v = bsum(tmp[(size_t)(j - KR) * BT + threadIdx.x], s_red);
+        for (int j = PE; j < k; ++j) {
+            const double v = bsum(tmp[(size_t)(j - PE) * BT + threadIdx.x], s_red);
             if (threadIdx.x == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
         }
-        for (int j = KR; j < k; ++j) tmp[(size_t)(j - KR) * BT + threadIdx.x] = 0.0;
+        for (int j = PE; j < k; ++j) tmp[(size_t)(j - PE) * BT + threadIdx.x] = 0.0;
         __syncthreads();
     }
     if (s_stat) {

@@ -147,6 +147,8 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     d->max_pent = max_pent;
     d->max_hept = max_hept;
     d->max_m = max_tri + max_pent + max_hept;
+    d->tmp_ld = (max_bundle - BQB_QP_REG_PARTIALS) * BQB_QP_THREADS;
+    if (d->tmp_ld < d->max_m) d->tmp_ld = d->max_m;
     d->tri_list = c->params.TriIneq;
     d->kg_list = BQB_SA_CHAINS;
     d->max_contrib = 3 * max_tri + 10 * max_pent + 21 * max_hept;
@@ -183,7 +185,7 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         {(void **)&c->p.gtest, sizeof(double) * N * d->max_m},
         {(void **)&c->p.eta, sizeof(double) * N * d->max_m},
         {(void **)&c->p.g, sizeof(double) * N * d->max_m},
-        {(void **)&c->p.tmpm, sizeof(double) * N * d->max_m},
+        {(void **)&c->p.tmpm, sizeof(double) * N * d->tmp_ld},
         {(void **)&c->p.tri, sizeof(TriCut) * N * max_tri},
         {(void **)&c->p.pent, sizeof(KgonCut) * N * max_pent},
         {(void **)&c->p.hept, sizeof(KgonCut) * N * max_hept},

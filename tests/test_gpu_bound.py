@@ -171,14 +171,17 @@ def test_opbt_gather_is_bitwise_the_reference_scatter():
     assert np.array_equal(L0, ref)
 
 
-def test_bundle_qp_matches_oracle_lambda_eta():
+@pytest.mark.parametrize("m,k", [(37, 6), (700, 24), (1500, 40)])
+def test_bundle_qp_matches_oracle_lambda_eta(m, k):
+    """k <= 16: all G'eta partial sums in registers; k = 24: partly in the
+    per-node scratch; k = 40: the LDS solver for the lambda subproblem.
+    m = 1500 with k = 40 stages [G gamma] through LDS in 16 row tiles."""
     n0 = 20
     SP = instance(n0, 0.5, "pm1", 2)
     xf = np.zeros((1, n0 - 1), np.int32)
     ctx = api.BoundContext(SP, max_nodes=1)
     basic(ctx, xf, xf)
-    rng = np.random.default_rng(3)
-    m, k = 37, 6
+    rng = np.random.default_rng(3 + k)
     G = rng.standard_normal((m, k))
     F = rng.standard_normal(k) * 3
     gam = np.abs(rng.standard_normal(m)) * (rng.random(m) < 0.5)

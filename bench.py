@@ -8,13 +8,17 @@ full SDPbound (FP64 IPM + bundle over triangle / pentagonal / heptagonal cuts,
 GW heuristic).  Every node receives the incumbent and the root gap of the
 instance's root bound, as a B&B worker does.
 
-One step = 2048 such nodes bounded through a context of 512 node slots
-(bqb_bound_stream: every launch covers the 512 slots, a finished node's slot
-is refilled after each outer round, as the B&B node queue runs it).  Inputs
-are uploaded inside the call (a few hundred KB; the timed region includes
-it).  value = node bounds per second.  The same nodes bounded as one fixed
-512-node batch (every launch waits for the slowest node) are reported in
-config.single_batch_bounds_per_s.  The roofline object covers the dominant
+One step = 16384 such nodes streamed through three slot groups of 512 node
+slots (bqb_bound_stream_multi: every launch covers one group's 512 slots, a
+finished node's slot is refilled after each outer round, as the B&B node
+queue runs it; each group has its own HIP stream and host thread, so one
+group's kernels fill the idle tail of another's launches).  The stream is
+long so that the ramp-down at its end, when few slots are still busy,
+weighs little, as in a node queue that is never empty.  Inputs are uploaded
+inside the call (a few hundred KB; the timed region includes it).
+value = node bounds per second.  For comparison, config reports the first
+2048 nodes streamed through a single group and the first 512 bounded as one
+fixed batch (every launch waits for the slowest node).  The roofline object covers the dominant
 kernel, the batched IPM: algorithmic FP64 flops (6 n^3 per iteration + n^3/3 per line-search
 Cholesky test, counted on the device) over the summed duration of its launches
 (HIP event pairs on the launch stream, during the timed steps).  The
@@ -84,7 +88,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nodes", type=int, default=512, help="node slots (subproblems per launch)")
-    ap.add_argument("--per-step", type=int, default=2048, help="nodes bounded per step")
+    ap.add_argument("--per-step", type=int, default=16384, help="nodes bounded per step")
+    ap.add_argument("--groups", type=int, default=3, help="slot groups (streams) of --nodes slots each")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -99,7 +104,7 @@ def main():
     from parallel_biqbin_maxcut_amd import api, instances as I
 
     SP, P, xf, sx, seeds = build_workload(api, I, args.seed, args.per_step)
-    ctx = api.BoundContext(SP, params=P, max_nodes=args.nodes)
+    ctx = api.BoundContext(SP, params=P, max_nodes=args.nodes, groups=args.groups)
     # root bound of the instance: incumbent and root gap the workers receive
     z = np.zeros((1, N0 - 1), np.int32)
     root = ctx.bound(z, z, LB=0.0, is_root=1, seeds=np.uint64(args.seed))
@@ -121,13 +126,20 @@ def main():
     if np.any(r["status"]):
         raise SystemExit(f"node bound failures: {np.unique(r['status'], return_counts=True)}")
 
-    # the first 512 nodes as one fixed batch (all launches wait for its slowest node)
-    single_batch = None
+    # comparisons: the first 2048 nodes through one slot group, the first 512
+    # as one fixed batch (all launches wait for its slowest node)
+    single_batch = single_group = None
     if not args.no_single:
+        one = api.BoundContext(SP, params=P, max_nodes=args.nodes)
+        k = min(2048, args.per_step)
         tb = time.perf_counter()
-        rb = ctx.bound(xf[: args.nodes], sx[: args.nodes], LB=LB, diff=diff, is_root=0, seeds=seeds[: args.nodes])
+        rg = one.bound(xf[:k], sx[:k], LB=LB, diff=diff, is_root=0, seeds=seeds[:k], stream=True)
+        single_group = round(k / (time.perf_counter() - tb), 2)
+        tb = time.perf_counter()
+        rb = one.bound(xf[: args.nodes], sx[: args.nodes], LB=LB, diff=diff, is_root=0, seeds=seeds[: args.nodes])
         single_batch = round(args.nodes / (time.perf_counter() - tb), 2)
-        if not np.array_equal(rb["bound"], r["bound"][: args.nodes]):
+        one.close()
+        if not (np.array_equal(rb["bound"], r["bound"][: args.nodes]) and np.array_equal(rg["bound"], r["bound"][:k])):
             raise SystemExit("streamed and batched bounds differ")
 
     ms_per_step = dt / args.steps * 1e3
@@ -152,7 +164,9 @@ def main():
                         "per launch with 0-40 variables fixed at random (sizes 60-100), FP64 IPM + bundle "
                         "with triangle/pentagonal/heptagonal cuts",
             "nodes_per_launch": args.nodes,
+            "slot_groups": args.groups,
             "nodes_per_step": args.per_step,
+            "one_group_2048_nodes_bounds_per_s": single_group,
             "single_batch_bounds_per_s": single_batch,
             "seed": args.seed,
             "incumbent": LB,

@@ -209,6 +209,18 @@ int bqb_bound_stream(bqb_ctx *c, int count, const int *xfixed, const int *solX, 
                      const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
                      double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
                      int *status, int *stats);
+/* Same through nctx (1..BQB_MAX_GROUPS) contexts of the same instance and
+ * parameters, each driven by its own host thread on its own HIP stream: a
+ * context takes the next node whenever one of its slots is free, so the
+ * kernels of one slot group run in the idle tail of the other's launches.
+ * Each launch still covers one context's slots.  Results are the same as
+ * bqb_bound_stream's (a node's results depend only on its inputs).  The
+ * contexts must not be used by other threads meanwhile. */
+#define BQB_MAX_GROUPS 8
+int bqb_bound_stream_multi(bqb_ctx *const *ctx, int nctx, int count, const int *xfixed, const int *solX,
+                           const double *LB, const double *diff, const int *is_root, const uint64_t *seeds,
+                           double *bound, double *fracsol, int *xbest, double *LB_out, int *improved,
+                           double *diff_out, int *status, int *stats);
 
 /* Incremental use (the node queue): push a node into a free slot (returns
  * the slot, -1 if none is free; tag >= 0 identifies it), advance every slot

@@ -272,6 +272,7 @@ def _declare_ctx(L):
     L.bqb_fct_eval.argtypes = [vp, C.c_int, N.dptr, C.c_int, N.iptr, C.c_int, N.iptr, C.c_int, N.iptr, N.dptr,
                                N.dptr, N.dptr, N.dptr, N.iptr]
     L.bqb_bound_stream.argtypes = L.bqb_bound_batch.argtypes
+    L.bqb_bound_stream_multi.argtypes = [C.POINTER(vp), C.c_int] + L.bqb_bound_batch.argtypes[1:]
     L.bqb_ctx_push.argtypes = [vp, C.c_longlong, N.iptr, N.iptr, C.c_double, C.c_double, C.c_int, C.c_uint64]
     L.bqb_ctx_advance.argtypes = [vp]
     L.bqb_ctx_pop.argtypes = [vp, C.POINTER(C.c_longlong), N.dptr, N.dptr, N.iptr, N.dptr, N.iptr, N.dptr,
@@ -292,22 +293,35 @@ def _declare_ctx(L):
 class BoundContext:
     """Device context for bounding batches of B&B nodes of one instance."""
 
-    def __init__(self, SP_L, params=None, max_nodes=1, max_bundle=0, max_tri=0, max_pent=0, max_hept=0):
+    def __init__(self, SP_L, params=None, max_nodes=1, max_bundle=0, max_tri=0, max_pent=0, max_hept=0,
+                 groups=1):
+        """groups > 1 adds slot groups of max_nodes slots each (contexts on
+        their own streams); streamed bounding (bound(stream=True)) then runs
+        through all groups with bqb_bound_stream_multi.  Everything else
+        acts on the first group."""
         L = N.lib()
         self.SP_L = np.asfortranarray(SP_L, dtype=np.float64)
         self.n0 = self.SP_L.shape[0]
         self.params = params if params is not None else make_params()
-        self.h = L.bqb_ctx_create(self.n0, _d(self.SP_L), C.byref(self.params), max_nodes, max_bundle,
-                                  max_tri, max_pent, max_hept)
-        if not self.h:
-            raise RuntimeError("bqb_ctx_create: " + L.bqb_last_error().decode())
+        self.handles = []
+        for _ in range(max(1, int(groups))):
+            h = L.bqb_ctx_create(self.n0, _d(self.SP_L), C.byref(self.params), max_nodes, max_bundle,
+                                 max_tri, max_pent, max_hept)
+            if not h:
+                err = L.bqb_last_error().decode()
+                self.close()
+                raise RuntimeError("bqb_ctx_create: " + err)
+            self.handles.append(h)
+        self.h = self.handles[0]
         self.ld = L.bqb_ctx_ld(self.h)
         self.max_nodes = max_nodes
+        self.groups = len(self.handles)
 
     def close(self):
-        if self.h:
-            N.lib().bqb_ctx_destroy(self.h)
-            self.h = None
+        for h in getattr(self, "handles", []):
+            N.lib().bqb_ctx_destroy(h)
+        self.handles = []
+        self.h = None
 
     def __del__(self):
         try:
@@ -335,11 +349,15 @@ class BoundContext:
         out = dict(bound=np.zeros(nodes), fracsol=np.zeros((nodes, B)), xbest=np.zeros((nodes, B), np.int32),
                    LB=np.zeros(nodes), improved=np.zeros(nodes, np.int32), diff=np.zeros(nodes),
                    status=np.zeros(nodes, np.int32), stats=np.zeros((nodes, 8), np.int32))
-        fn = N.lib().bqb_bound_stream if stream else N.lib().bqb_bound_batch
-        N.check(fn(
-            self.h, nodes, _i(xf), _i(sx), _d(lb), _d(df), _i(rt), sd.ctypes.data_as(C.POINTER(C.c_uint64)),
-            _d(out["bound"]), _d(out["fracsol"]), _i(out["xbest"]), _d(out["LB"]), _i(out["improved"]),
-            _d(out["diff"]), _i(out["status"]), _i(out["stats"])))
+        args = (nodes, _i(xf), _i(sx), _d(lb), _d(df), _i(rt), sd.ctypes.data_as(C.POINTER(C.c_uint64)),
+                _d(out["bound"]), _d(out["fracsol"]), _i(out["xbest"]), _d(out["LB"]), _i(out["improved"]),
+                _d(out["diff"]), _i(out["status"]), _i(out["stats"]))
+        if stream and self.groups > 1:
+            hs = (C.c_void_p * self.groups)(*self.handles)
+            N.check(N.lib().bqb_bound_stream_multi(hs, self.groups, *args))
+        else:
+            fn = N.lib().bqb_bound_stream if stream else N.lib().bqb_bound_batch
+            N.check(fn(self.h, *args))
         out["ms"] = N.lib().bqb_ctx_last_ms(self.h)
         return out
 
@@ -395,15 +413,22 @@ class BoundContext:
     def profile_ipm(self, max_launches):
         """Time every IPM launch (event pairs) and count its algorithmic
         flops on the device, for up to max_launches launches; 0 stops."""
-        N.check(N.lib().bqb_ctx_profile_ipm(self.h, max_launches))
+        for h in self.handles:
+            N.check(N.lib().bqb_ctx_profile_ipm(h, max_launches))
 
     def ipm_stats(self):
-        """(launches, summed ms, algorithmic flops) since profiling started."""
-        k = C.c_int()
-        ms = C.c_double()
-        fl = C.c_double()
-        N.check(N.lib().bqb_ctx_ipm_stats(self.h, C.byref(k), C.byref(ms), C.byref(fl)))
-        return k.value, ms.value, fl.value
+        """(launches, summed ms, algorithmic flops) since profiling started,
+        over all slot groups."""
+        tot = [0, 0.0, 0.0]
+        for h in self.handles:
+            k = C.c_int()
+            ms = C.c_double()
+            fl = C.c_double()
+            N.check(N.lib().bqb_ctx_ipm_stats(h, C.byref(k), C.byref(ms), C.byref(fl)))
+            tot[0] += k.value
+            tot[1] += ms.value
+            tot[2] += fl.value
+        return tuple(tot)
 
     def matrix(self, buf, node, n):
         a = self.get(buf, node, np.float64, self.ld * self.ld).reshape((self.ld, self.ld), order="F")

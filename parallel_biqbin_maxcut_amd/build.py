@@ -24,7 +24,7 @@ HIPCC_FLAGS = [
     "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
 CC_FLAGS = [
-    "-O2", "-fPIC", "-std=c11", "-D_POSIX_C_SOURCE=200809L", "-D__HIP_PLATFORM_AMD__",
+    "-O2", "-fPIC", "-pthread", "-std=c11", "-D_POSIX_C_SOURCE=200809L", "-D__HIP_PLATFORM_AMD__",
     "-Wall", "-Wextra", "-Wno-unused-parameter",
     "-I" + os.path.join(ROCM, "include"), "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
@@ -60,7 +60,8 @@ def build(jobs=8, verbose=False):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+         + ["-lpthread"])
     os.replace(tmp, LIB)
     # command-line solver, linked against the library next to it
     _run(["gcc"] + CC_FLAGS + [CLI_SRC, "-o", CLI + ".tmp", "-L" + HERE, "-lbiqbin_gpu",

@@ -17,7 +17,7 @@
 #include "bqb_kernels.h"
 #include "bqb_runtime.h"
 
-static char g_err[512];
+static _Thread_local char g_err[512];
 static int g_device = -1;
 
 void bqb_set_error(const char *fmt, ...)

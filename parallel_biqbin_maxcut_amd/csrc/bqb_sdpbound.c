@@ -16,6 +16,7 @@
  */
 #include <stdint.h>
 #include <stdio.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -598,54 +599,162 @@ int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, doub
 
 /* Bound `count` nodes (any number) through the context's slots, refilling
  * free slots after every outer round; outputs in input order. */
-int bqb_bound_stream(bqb_ctx *c, int count, const int *xfixed, const int *solX, const double *LB,
-                     const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
-                     double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
-                     int *status, int *stats)
+/* A stream of nodes shared by one or more contexts: each context takes the
+ * next node index whenever it has a free slot, and writes each result at the
+ * node's index. */
+struct stream_job {
+    pthread_mutex_t mu;
+    int count, next, failed;
+    const int *xfixed, *solX;
+    const double *LB, *diff;
+    const int *is_root;
+    const uint64_t *seeds;
+    double *bound, *fracsol;
+    int *xbest;
+    double *LB_out;
+    int *improved;
+    double *diff_out;
+    int *status, *stats;
+};
+
+static int stream_take(struct stream_job *j)
 {
-    if (count < 0) { bqb_set_error("negative node count"); return -1; }
+    pthread_mutex_lock(&j->mu);
+    const int i = (j->failed || j->next >= j->count) ? -1 : j->next++;
+    pthread_mutex_unlock(&j->mu);
+    return i;
+}
+
+static int stream_failed(struct stream_job *j)
+{
+    pthread_mutex_lock(&j->mu);
+    const int f = j->failed;
+    pthread_mutex_unlock(&j->mu);
+    return f;
+}
+
+/* Run the context until the shared pool is empty and its slots drained. */
+static int stream_run(bqb_ctx *c, struct stream_job *j)
+{
     engine_reset(c);
     const size_t B = (size_t)(c->d.n0 - 1);
     double *frac = malloc(sizeof(double) * (B ? B : 1));
     int *xb = malloc(sizeof(int) * (B ? B : 1));
     if (!frac || !xb) { free(frac); free(xb); bqb_set_error("calloc"); return -1; }
-    float ms = 0.f;
-    hipEventRecord(c->ev0, c->stream);
-    int next = 0, got = 0, rc = -1;
-    while (got < count) {
-        while (next < count && c->nfree > 0) {
-            if (bqb_ctx_push(c, next, xfixed + next * B, solX + next * B, LB ? LB[next] : 0.0,
-                             diff ? diff[next] : 0.0, is_root ? is_root[next] : 0,
-                             seeds ? seeds[next] : (uint64_t)next) < 0)
+    int rc = -1, pool = 1;
+    for (;;) {
+        while (pool && c->nfree > 0) {
+            const int i = stream_take(j);
+            if (i < 0) { pool = 0; break; }
+            if (bqb_ctx_push(c, i, j->xfixed + i * B, j->solX + i * B, j->LB ? j->LB[i] : 0.0,
+                             j->diff ? j->diff[i] : 0.0, j->is_root ? j->is_root[i] : 0,
+                             j->seeds ? j->seeds[i] : (uint64_t)i) < 0)
                 goto out;
-            ++next;
         }
+        if (bqb_ctx_busy(c) == 0) break;
         if (bqb_ctx_advance(c)) goto out;
         long long t;
         double bd, lbo, dfo;
         int imp, st, sts[8];
         while (bqb_ctx_pop(c, &t, &bd, frac, xb, &lbo, &imp, &dfo, &st, sts)) {
-            if (bound) bound[t] = bd;
-            if (fracsol) memcpy(fracsol + t * B, frac, sizeof(double) * B);
-            if (xbest) memcpy(xbest + t * B, xb, sizeof(int) * B);
-            if (LB_out) LB_out[t] = lbo;
-            if (improved) improved[t] = imp;
-            if (diff_out) diff_out[t] = dfo;
-            if (status) status[t] = st;
-            if (stats) memcpy(stats + 8 * t, sts, sizeof sts);
-            ++got;
+            if (j->bound) j->bound[t] = bd;
+            if (j->fracsol) memcpy(j->fracsol + t * B, frac, sizeof(double) * B);
+            if (j->xbest) memcpy(j->xbest + t * B, xb, sizeof(int) * B);
+            if (j->LB_out) j->LB_out[t] = lbo;
+            if (j->improved) j->improved[t] = imp;
+            if (j->diff_out) j->diff_out[t] = dfo;
+            if (j->status) j->status[t] = st;
+            if (j->stats) memcpy(j->stats + 8 * t, sts, sizeof sts);
         }
+        if (stream_failed(j)) { bqb_set_error("stopped: another context failed"); goto out; }
     }
     rc = 0;
 out:
     free(frac);
     free(xb);
+    return rc;
+}
+
+struct stream_worker {
+    struct stream_job *job;
+    bqb_ctx *c;
+    int rc;
+    char err[512];
+};
+
+static void *stream_thread(void *arg)
+{
+    struct stream_worker *w = arg;
+    w->rc = bqb_ensure_init();
+    if (w->rc == 0) w->rc = stream_run(w->c, w->job);
+    if (w->rc) {
+        snprintf(w->err, sizeof w->err, "%s", bqb_last_error());
+        pthread_mutex_lock(&w->job->mu);
+        w->job->failed = 1;
+        pthread_mutex_unlock(&w->job->mu);
+    }
+    return NULL;
+}
+
+int bqb_bound_stream_multi(bqb_ctx *const *ctx, int nctx, int count, const int *xfixed, const int *solX,
+                           const double *LB, const double *diff, const int *is_root, const uint64_t *seeds,
+                           double *bound, double *fracsol, int *xbest, double *LB_out, int *improved,
+                           double *diff_out, int *status, int *stats)
+{
+    if (count < 0) { bqb_set_error("negative node count"); return -1; }
+    if (nctx < 1 || nctx > BQB_MAX_GROUPS || !ctx) { bqb_set_error("1..%d contexts", BQB_MAX_GROUPS); return -1; }
+    for (int i = 0; i < nctx; ++i) {
+        if (!ctx[i] || ctx[i]->d.n0 != ctx[0]->d.n0) { bqb_set_error("contexts of different instances"); return -1; }
+        for (int k = 0; k < i; ++k)
+            if (ctx[k] == ctx[i]) { bqb_set_error("a context appears twice"); return -1; }
+    }
+    struct stream_job j = {.count = count, .xfixed = xfixed, .solX = solX, .LB = LB, .diff = diff,
+                           .is_root = is_root, .seeds = seeds, .bound = bound, .fracsol = fracsol,
+                           .xbest = xbest, .LB_out = LB_out, .improved = improved, .diff_out = diff_out,
+                           .status = status, .stats = stats};
+    pthread_mutex_init(&j.mu, NULL);
+    struct stream_worker w[BQB_MAX_GROUPS];
+    pthread_t th[BQB_MAX_GROUPS];
+    int started = 0;
+    for (int i = 1; i < nctx; ++i) {
+        w[i] = (struct stream_worker){.job = &j, .c = ctx[i]};
+        if (pthread_create(&th[i], NULL, stream_thread, &w[i])) {
+            pthread_mutex_lock(&j.mu);
+            j.failed = 1;
+            pthread_mutex_unlock(&j.mu);
+            break;
+        }
+        started = i;
+    }
+    hipEventRecord(ctx[0]->ev0, ctx[0]->stream);
+    int rc = stream_run(ctx[0], &j);
+    if (rc) {
+        pthread_mutex_lock(&j.mu);
+        j.failed = 1;
+        pthread_mutex_unlock(&j.mu);
+    }
+    for (int i = 1; i <= started; ++i) {
+        pthread_join(th[i], NULL);
+        if (w[i].rc && !rc) { rc = -1; bqb_set_error("%s", w[i].err); }
+    }
+    if (!rc && started < nctx - 1) { rc = -1; bqb_set_error("pthread_create failed"); }
+    pthread_mutex_destroy(&j.mu);
     if (rc) return rc;
-    hipEventRecord(c->ev1, c->stream);
-    if (bqb_check(hipEventSynchronize(c->ev1), "sync")) return -1;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    c->last_ms = ms;
+    hipEventRecord(ctx[0]->ev1, ctx[0]->stream);
+    if (bqb_check(hipEventSynchronize(ctx[0]->ev1), "sync")) return -1;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ctx[0]->ev0, ctx[0]->ev1);
+    ctx[0]->last_ms = ms;
     return 0;
+}
+
+int bqb_bound_stream(bqb_ctx *c, int count, const int *xfixed, const int *solX, const double *LB,
+                     const double *diff, const int *is_root, const uint64_t *seeds, double *bound,
+                     double *fracsol, int *xbest, double *LB_out, int *improved, double *diff_out,
+                     int *status, int *stats)
+{
+    return bqb_bound_stream_multi(&c, 1, count, xfixed, solX, LB, diff, is_root, seeds, bound, fracsol,
+                                  xbest, LB_out, improved, diff_out, status, stats);
 }
 
 int bqb_bound_batch(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,

@@ -369,3 +369,9 @@ def test_stream_through_few_slots_equals_batch_bitwise():
     for k in ("bound", "fracsol", "xbest", "LB", "improved", "diff", "stats"):
         assert np.array_equal(a[k], b[k]), k
     assert len(set(a["stats"][:, 2])) > 1          # the nodes ran different numbers of rounds
+    # three slot groups of 5 on their own streams and host threads
+    groups = api.BoundContext(SP, params=P, max_nodes=5, groups=3)
+    c = groups.bound(xf, sx, LB=LB, diff=diff, seeds=seeds, stream=True)
+    for k in ("bound", "fracsol", "xbest", "LB", "improved", "diff", "stats"):
+        assert np.array_equal(a[k], c[k]), k
+    groups.close()

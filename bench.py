@@ -72,6 +72,47 @@ def cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample):
                       f"{dt:.1f} s wall on {cores} threads"}, bounds, status
 
 
+def time_to_proof(api, I, seeds, cpu_seeds):
+    """Full B&B to the proven optimum, one node at a time (batch 1) and with
+    a 64-slot node queue, on G(60, 0.5) with unit weights, triangle cuts only
+    (the reference's params with pentagonal and heptagonal cuts off).  The
+    CPU oracle's sequential B&B runs on the first instances for comparison."""
+    P = api.make_params(include_Pent=0, include_Hepta=0)
+    out = {"workload": f"G(60, 0.5) unit weights, triangles only, seeds {seeds[0]}-{seeds[-1]}",
+           "optimum": [], "nodes_batch1": [], "seconds_batch1": [], "seconds_batch64": []}
+    Ls = []
+    for sd in seeds:
+        L = api.maxcut_L(I.gnp_adj(60, 0.5, sd, weights="unit"))
+        Ls.append(L)
+        st1, _ = api.solve(L, params=P, seed=2020, batch=1)
+        st64, _ = api.solve(L, params=P, seed=2020, batch=64)
+        if st1.status or st64.status or st1.best != st64.best:
+            raise SystemExit(f"time-to-proof instance {sd}: status {st1.status}/{st64.status}, "
+                             f"optimum {st1.best} vs {st64.best}")
+        out["optimum"].append(st1.best)
+        out["nodes_batch1"].append(int(st1.nodes))
+        out["seconds_batch1"].append(round(st1.seconds, 4))
+        out["seconds_batch64"].append(round(st64.seconds, 4))
+    out["median_s_batch1"] = float(np.median(out["seconds_batch1"]))
+    out["median_s_batch64"] = float(np.median(out["seconds_batch64"]))
+    if cpu_seeds:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        d = {f: getattr(P, f) for f, _ in type(P)._fields_}
+        d["TriIneq"] = 10 * 60                     # src/process_input.c:135-136
+        cs = []
+        for i in range(cpu_seeds):
+            t0 = time.perf_counter()
+            ob, _ = O.bab(Ls[i], O.default_params(**d), seed=2020, batch=1)
+            cs.append(round(time.perf_counter() - t0, 4))
+            if ob.best != out["optimum"][i]:
+                raise SystemExit(f"time-to-proof instance {seeds[i]}: CPU oracle optimum {ob.best}, "
+                                 f"GPU {out['optimum'][i]}")
+        out["cpu_oracle_seconds_batch1"] = cs
+        out["cpu_oracle_cores"] = 1
+    return out
+
+
 def traffic_per_launch():
     """HBM bytes per IPM launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload, or None."""
@@ -93,6 +134,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-proof", action="store_true", help="skip the time-to-proof instances")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-batch comparison after the timed steps (profiling runs)")
     args = ap.parse_args()
@@ -189,6 +231,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_proof:
+        line["config"]["time_to_proof"] = time_to_proof(api, I, list(range(1, 21)), 0 if args.no_cpu else 3)
     if not args.no_cpu:
         sample = min(args.cpu_sample, args.per_step)
         cb, cbounds, cstatus = cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample)

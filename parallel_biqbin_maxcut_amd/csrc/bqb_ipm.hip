@@ -450,6 +450,9 @@ extern "C" hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int
     case 1607: return launch_ipm<16, 7>(*a, nodes, s);
     case 3204: return launch_ipm<32, 4>(*a, nodes, s);
     case 3205: return launch_ipm<32, 5>(*a, nodes, s);
+    case 3206: return launch_ipm<32, 6>(*a, nodes, s);
+    case 3207: return launch_ipm<32, 7>(*a, nodes, s);
+    case 3208: return launch_ipm<32, 8>(*a, nodes, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -305,6 +305,9 @@ extern "C" hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, in
     case 1607: return launch_heur<16, 7>(*d, *p, s);
     case 3204: return launch_heur<32, 4>(*d, *p, s);
     case 3205: return launch_heur<32, 5>(*d, *p, s);
+    case 3206: return launch_heur<32, 6>(*d, *p, s);
+    case 3207: return launch_heur<32, 7>(*d, *p, s);
+    case 3208: return launch_heur<32, 8>(*d, *p, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -416,12 +416,13 @@ using namespace bqb;
 // Picks the register-tile shape for the largest n in the batch.
 static int g_ipm_tile = 0;   // 0: by size; else a forced tile (tuning)
 
-// Forces the IPM register tile (P*100 + R: 1604 ... 3205) for later
+// Forces the IPM register tile (P*100 + R: 1604 ... 3208) for later
 // contexts and batches whose size it covers; 0 restores the size rule.
 extern "C" int bqb_set_ipm_tile(int tile)
 {
     switch (tile) {
     case 0: case 1604: case 1605: case 1606: case 1607: case 3204: case 3205:
+    case 3206: case 3207: case 3208:
         g_ipm_tile = tile;
         return 0;
     default:
@@ -438,6 +439,11 @@ extern "C" int bqb_ipm_tile_for(int nmax)
     if (nmax <= 112) return 1607;
     if (nmax <= 128) return 3204;
     if (nmax <= 160) return 3205;
+    // above 160 the 32 x 32 tile holds more registers than a 1024-thread
+    // block has per lane: the compiler spills part of the tile
+    if (nmax <= 192) return 3206;
+    if (nmax <= 224) return 3207;
+    if (nmax <= 256) return 3208;
     return 0;
 }
 

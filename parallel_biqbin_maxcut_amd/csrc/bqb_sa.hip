@@ -44,6 +44,13 @@ __device__ __forceinline__ double xs(const double *Xp, int a, int b)
     return Xp[hi * (hi + 1) / 2 + lo];
 }
 
+// the same element read from the node's X slab (X too large for LDS)
+__device__ __forceinline__ double xg(const double *X, int ld, int a, int b)
+{
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return X[lo + (size_t)hi * ld];
+}
+
 // A chain's permutation: in LDS (16-bit entries) when it fits next to the
 // packed X, otherwise in the node's global scratch.
 template <bool LDSP>
@@ -58,10 +65,11 @@ struct PermRef {
     }
 };
 
-template <int K, bool LDSP>
+template <int K, bool XL, bool LDSP>
 __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
 {
-    extern __shared__ double s_X[];            // packed triangle of X, then the LDS permutations
+    // packed triangle of X (XL), then the LDS permutations (LDSP)
+    extern __shared__ double s_X[];
     __shared__ double s_t0;
     __shared__ int s_cnt[ST];
     __shared__ int s_qn;
@@ -85,7 +93,8 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
         if (threadIdx.x == 0 && K == 5) sp->viol5 = 1e9;
         return;
     }
-    for (int e = threadIdx.x; e < n * (n + 1) / 2; e += ST) {
+    auto xa = [&](int a, int b) { return XL ? xs(s_X, a, b) : xg(X, ld, a, b); };
+    for (int e = threadIdx.x; XL && e < n * (n + 1) / 2; e += ST) {
         int hi = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
         while (hi * (hi + 1) / 2 > e) --hi;
         while ((hi + 1) * (hi + 2) / 2 <= e) ++hi;
@@ -112,7 +121,7 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
         const uint64_t key = bqb_key(rkey, (uint64_t)c);
         uint64_t ctr = 0;
         PermRef<LDSP> perm;
-        perm.s = reinterpret_cast<unsigned short *>(s_X + (size_t)ld * (ld + 1) / 2) + (size_t)c * ld;
+        perm.s = reinterpret_cast<unsigned short *>(s_X + (XL ? (size_t)ld * (ld + 1) / 2 : 0)) + (size_t)c * ld;
         perm.g = p.perm + ((size_t)node * BQB_SA_CHAINS + c) * ld;
         for (int i = 0; i < n; ++i) perm.set(i, i);
         for (int i = n - 1; i > 0; --i) {
@@ -133,7 +142,7 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
 #pragma unroll
         for (int i = 0; i < K; ++i)
 #pragma unroll
-            for (int j = i + 1; j < K; ++j) sol += 2 * (e[j] * e[i]) * xs(s_X, pk.get(j), pk.get(i));
+            for (int j = i + 1; j < K; ++j) sol += 2 * (e[j] * e[i]) * xa(pk.get(j), pk.get(i));
         double cost = 1e9;
         int not_done = 1, stages = 0, status = 0;
         while (not_done > 0) {
@@ -149,15 +158,15 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
                 double delta = 0.0;
 #pragma unroll
                 for (int i = 0; i < K; ++i)
-                    delta += (double)(e[i] * ei1) * (xs(s_X, pk.get(i), p2) - xs(s_X, pk.get(i), p1));
+                    delta += (double)(e[i] * ei1) * (xa(pk.get(i), p2) - xa(pk.get(i), p1));
                 if (i2 < K) {
                     const int ei2 = kgon_e(type, i2);
 #pragma unroll
                     for (int i = 0; i < K; ++i)
-                        delta += (double)(e[i] * ei2) * (xs(s_X, pk.get(i), p1) - xs(s_X, pk.get(i), p2));
+                        delta += (double)(e[i] * ei2) * (xa(pk.get(i), p1) - xa(pk.get(i), p2));
                 }
                 delta *= 2.0;
-                const double tv1 = 2.0 - 2.0 * xs(s_X, p2, p1);
+                const double tv1 = 2.0 - 2.0 * xa(p2, p1);
                 double tv2 = 1.0;
                 if (i2 < K) tv2 += -2 * (kgon_e(type, i1) * kgon_e(type, i2)) + 1.0;
                 delta += tv1 * tv2;
@@ -300,22 +309,29 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
 
 }  // namespace bqb
 
+template <int K>
+static void launch_sa(const BatchDims *d, const BatchPtrs *p, bool xl, bool lp, size_t lds, hipStream_t s)
+{
+    if (xl && lp) hipLaunchKernelGGL((bqb::sa_kernel<K, true, true>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+    else if (xl) hipLaunchKernelGGL((bqb::sa_kernel<K, true, false>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+    else if (lp) hipLaunchKernelGGL((bqb::sa_kernel<K, false, true>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+    else hipLaunchKernelGGL((bqb::sa_kernel<K, false, false>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
+}
+
 extern "C" hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStream_t s)
 {
     const size_t lds_x = sizeof(double) * (size_t)d->ld * (d->ld + 1) / 2;
     const int chains = (k == 5) ? d->sa_chains5 : d->sa_chains7;
     const size_t lds_p = sizeof(unsigned short) * (size_t)chains * d->ld;
-    // the chains' permutations join the packed X in LDS while both fit in
-    // the 160 KB of a CU (less the kernel's static LDS); otherwise they stay
-    // in the node's global scratch
-    const bool in_lds = lds_x + lds_p <= 156 * 1024;
-    const size_t lds = in_lds ? lds_x + lds_p : lds_x;
-    if (k == 5) {
-        if (in_lds) hipLaunchKernelGGL((bqb::sa_kernel<5, true>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
-        else hipLaunchKernelGGL((bqb::sa_kernel<5, false>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
-    } else {
-        if (in_lds) hipLaunchKernelGGL((bqb::sa_kernel<7, true>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
-        else hipLaunchKernelGGL((bqb::sa_kernel<7, false>), dim3(d->nodes), dim3(bqb::ST), lds, s, *d, *p);
-    }
+    // X is staged in LDS as a packed triangle while it fits in the 160 KB of
+    // a CU (less the kernel's static LDS), else read from the node's slab;
+    // the chains' permutations join it in LDS while both fit, otherwise they
+    // stay in the node's global scratch
+    const size_t cap = 156 * 1024;
+    const bool xl = lds_x <= cap;
+    const bool lp = (xl ? lds_x : 0) + lds_p <= cap;
+    const size_t lds = (xl ? lds_x : 0) + (lp ? lds_p : 0);
+    if (k == 5) launch_sa<5>(d, p, xl, lp, lds, s);
+    else launch_sa<7>(d, p, xl, lp, lds, s);
     return hipGetLastError();
 }

@@ -120,7 +120,7 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     if (!tile) { bqb_set_error("n0 = %d exceeds the largest compiled tile", n0); return NULL; }
     /* the heuristic's 1-opt runs one hyperplane per wave: 16 waves (32 x 32
      * threads) where the 32 x R tile covers n0 */
-    const int heur_tile = (n0 <= 128) ? 3204 : 3205;
+    const int heur_tile = (n0 <= 128) ? 3204 : (n0 <= 160) ? 3205 : (n0 <= 192) ? 3206 : (n0 <= 224) ? 3207 : 3208;
     const int tile_ld = (tile / 100) * (tile % 100), heur_ld = (heur_tile / 100) * (heur_tile % 100);
     bqb_ctx *c = calloc(1, sizeof(bqb_ctx));
     if (!c) { bqb_set_error("calloc"); return NULL; }

@@ -108,9 +108,10 @@ def test_triangle_separation_matches_oracle(n0, thr):
     assert [tuple(r) for r in cuts] == [(i, j, k, t) for (i, j, k, t, _) in lst]
 
 
-@pytest.mark.parametrize("k", [5, 7])
-def test_kgonal_separation_matches_oracle(k):
-    n0 = 40
+@pytest.mark.parametrize("k,n0", [(5, 40), (7, 40), (5, 250), (7, 250)])
+def test_kgonal_separation_matches_oracle(k, n0):
+    """n0 = 250: the packed triangle of X exceeds LDS, the chains read X
+    from its slab."""
     SP = instance(n0, 0.5, "pm1", 9)
     xf = np.zeros((1, n0 - 1), np.int32)
     P = api.make_params()
@@ -235,6 +236,22 @@ def test_root_node_three_rounds_match_oracle():
     assert (s[0], s[1], s[2], s[5]) == (o.prune, o.giveup, o.outer_iters, o.max_cuts)
     assert abs(r["bound"][0] - o.bound) <= 1e-6 * abs(o.bound), (r["bound"][0], o.bound)
     assert np.array_equal(r["xbest"][0], outs[0][2])
+
+
+def test_cfg3_root_node_three_rounds_match_oracle():
+    """G(250, 0.1) with weights +-1..+-10 on the 32 x 8 tiles: three outer
+    rounds with triangle cuts, same decisions, bound within 1e-6."""
+    n0 = 250
+    SP = instance(n0, 0.1, "int10", 3)
+    P = api.make_params(include_Pent=0, include_Hepta=0, max_outer_iter=3, extra_iter=0)
+    xf = np.zeros((1, n0 - 1), np.int32)
+    r, outs = _bound_pair(SP, P, xf, xf, 0.0, 0.0, 1, 5)
+    o = outs[0][0]
+    assert r["status"][0] == 0 and o.status == 0
+    assert r["LB"][0] == o.LB
+    s = r["stats"][0]
+    assert (s[0], s[1], s[2], s[5]) == (o.prune, o.giveup, o.outer_iters, o.max_cuts)
+    assert abs(r["bound"][0] - o.bound) <= 1e-6 * abs(o.bound), (r["bound"][0], o.bound)
 
 
 def test_root_node_full_run_close_to_oracle():

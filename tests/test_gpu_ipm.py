@@ -46,9 +46,15 @@ def test_ipm_batch_unit_weights_cfg1_root():
     compare([L], r, trace=True)
 
 
-@pytest.mark.parametrize("n0", [2, 3, 17, 64, 65, 97, 128, 151])
+@pytest.mark.parametrize("n0", [2, 3, 17, 64, 65, 97, 128, 151, 161, 200, 250])
 def test_ipm_batch_sizes_and_tiles(n0):
     Ls = sub_matrices(n0, 0.5, "int10", n0, 3, min(5, n0 - 1))
+    compare(Ls, api.ipm_batch(Ls))
+
+
+def test_ipm_batch_cfg3_like():
+    """G(250, 0.1), weights +-1..+-10: sizes 210-250 (tiles 32 x 7, 32 x 8)."""
+    Ls = sub_matrices(250, 0.1, "int10", 31, 3, 40)
     compare(Ls, api.ipm_batch(Ls))
 
 

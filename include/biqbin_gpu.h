@@ -235,6 +235,30 @@ int bqb_ctx_pop(bqb_ctx *c, long long *tag, double *bound, double *fracsol, int 
 int bqb_ctx_busy(const bqb_ctx *c);       /* occupied slots */
 int bqb_ctx_capacity(const bqb_ctx *c);   /* max_nodes */
 
+/* Trajectory capture (the training data of a learned warm start; no
+ * counterpart in the reference).  While capture is on, every node that
+ * finishes leaves a record: its outer rounds (after each: f, bundle step t,
+ * bound and the triangle / pentagonal / heptagonal cut counts; the rounds of
+ * bounding.c:170-336), and its final cut set with the multipliers gamma
+ * (laid out [triangles | pentagonals | heptagonals] as bounding.c:268-275),
+ * for which f = fct_eval(gamma).  Cut records: tri 4 ints (i, j, k, type),
+ * pentagonal and heptagonal 8 ints (type, then 5 or 7 vertices, padded);
+ * vertices 0-based in the node's subproblem.  Records persist until
+ * cleared; their pointers are owned by the context. */
+typedef struct bqb_trajectory {
+    long long tag;
+    int n, rounds, ntri, npent, nhept, status;
+    double f, t, bound, fixedvalue;
+    const double *round_f, *round_t, *round_bound;   /* [rounds] */
+    const int *round_cuts;                           /* [rounds*3] */
+    const int *tri, *pent, *hept;                    /* [ntri*4], [npent*8], [nhept*8] */
+    const double *gamma;                             /* [ntri+npent+nhept] */
+} bqb_trajectory;
+int bqb_ctx_capture(bqb_ctx *c, int on);             /* start (no node in flight) / stop */
+int bqb_ctx_trajectory_count(const bqb_ctx *c);
+int bqb_ctx_trajectory(const bqb_ctx *c, int i, bqb_trajectory *out);
+void bqb_ctx_trajectory_clear(bqb_ctx *c);
+
 /* Stepwise use of a context (tests and diagnostics): load a batch into
  * slots 0..nodes-1, run one phase (0-14 control phases; 100 basic IPM,
  * 101 IPM on L0, 102 heuristic,

@@ -154,6 +154,25 @@ PH = dict(SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OU
           HEUR=102, TRISEP=103, PENT=104, HEPT=105, CSR=106, OPBT=107, QP=108, STEP=109)
 
 
+class Trajectory(C.Structure):
+    """bqb_trajectory (include/biqbin_gpu.h)."""
+    _fields_ = [("tag", C.c_longlong), ("n", C.c_int), ("rounds", C.c_int), ("ntri", C.c_int),
+                ("npent", C.c_int), ("nhept", C.c_int), ("status", C.c_int), ("f", C.c_double),
+                ("t", C.c_double), ("bound", C.c_double), ("fixedvalue", C.c_double),
+                ("round_f", C.POINTER(C.c_double)), ("round_t", C.POINTER(C.c_double)),
+                ("round_bound", C.POINTER(C.c_double)), ("round_cuts", C.POINTER(C.c_int)),
+                ("tri", C.POINTER(C.c_int)), ("pent", C.POINTER(C.c_int)), ("hept", C.POINTER(C.c_int)),
+                ("gamma", C.POINTER(C.c_double))]
+
+
+def _arr(ptr, count, dtype, shape=None):
+    if count == 0:
+        a = np.zeros(0, dtype=dtype)
+    else:
+        a = np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+    return a.reshape(shape) if shape is not None else a
+
+
 class SolveStats(C.Structure):
     _fields_ = [("best", C.c_double), ("root_bound", C.c_double), ("nodes", C.c_longlong),
                 ("batches", C.c_longlong), ("status", C.c_int), ("stopped", C.c_int),
@@ -273,6 +292,11 @@ def _declare_ctx(L):
                                N.dptr, N.dptr, N.dptr, N.iptr]
     L.bqb_bound_stream.argtypes = L.bqb_bound_batch.argtypes
     L.bqb_bound_stream_multi.argtypes = [C.POINTER(vp), C.c_int] + L.bqb_bound_batch.argtypes[1:]
+    L.bqb_ctx_capture.argtypes = [vp, C.c_int]
+    L.bqb_ctx_trajectory_count.argtypes = [vp]
+    L.bqb_ctx_trajectory.argtypes = [vp, C.c_int, C.POINTER(Trajectory)]
+    L.bqb_ctx_trajectory_clear.argtypes = [vp]
+    L.bqb_ctx_trajectory_clear.restype = None
     L.bqb_ctx_push.argtypes = [vp, C.c_longlong, N.iptr, N.iptr, C.c_double, C.c_double, C.c_int, C.c_uint64]
     L.bqb_ctx_advance.argtypes = [vp]
     L.bqb_ctx_pop.argtypes = [vp, C.POINTER(C.c_longlong), N.dptr, N.dptr, N.iptr, N.dptr, N.iptr, N.dptr,
@@ -409,6 +433,34 @@ class BoundContext:
 
     def sync(self):
         N.check(N.lib().bqb_ctx_sync(self.h))
+
+    def capture(self, on=True):
+        """Record the trajectory of every node that finishes (all groups)."""
+        for h in self.handles:
+            N.check(N.lib().bqb_ctx_capture(h, 1 if on else 0))
+
+    def trajectories(self, clear=True):
+        """The captured records as dicts of numpy arrays, ordered by tag."""
+        L = N.lib()
+        out = []
+        for h in self.handles:
+            for i in range(L.bqb_ctx_trajectory_count(h)):
+                t = Trajectory()
+                N.check(L.bqb_ctx_trajectory(h, i, C.byref(t)))
+                R, m = t.rounds, t.ntri + t.npent + t.nhept
+                out.append(dict(
+                    tag=t.tag, n=t.n, rounds=R, status=t.status, f=t.f, t=t.t, bound=t.bound,
+                    fixedvalue=t.fixedvalue, round_f=_arr(t.round_f, R, np.float64),
+                    round_t=_arr(t.round_t, R, np.float64), round_bound=_arr(t.round_bound, R, np.float64),
+                    round_cuts=_arr(t.round_cuts, 3 * R, np.int32, (R, 3)),
+                    tri=_arr(t.tri, 4 * t.ntri, np.int32, (t.ntri, 4)),
+                    pent=_arr(t.pent, 8 * t.npent, np.int32, (t.npent, 8)),
+                    hept=_arr(t.hept, 8 * t.nhept, np.int32, (t.nhept, 8)),
+                    gamma=_arr(t.gamma, m, np.float64)))
+            if clear:
+                L.bqb_ctx_trajectory_clear(h)
+        out.sort(key=lambda r: r["tag"])
+        return out
 
     def profile_ipm(self, max_launches):
         """Time every IPM launch (event pairs) and count its algorithmic

@@ -70,6 +70,15 @@ struct bqb_ctx {
     /* finished nodes not yet popped (FIFO) */
     struct done_node *done;
     int ndone, done_head, done_cap;
+    /* trajectory capture: per-slot log of completed outer rounds, and the
+     * records of the nodes that finished while capture was on */
+    int capture;
+    int tr_rmax;               /* rounds per slot log */
+    double *tr_f, *tr_t, *tr_bound;   /* [max_nodes*tr_rmax] */
+    int *tr_cuts;              /* [max_nodes*tr_rmax*3] */
+    int *tr_n;                 /* [max_nodes] rounds logged */
+    bqb_trajectory *traj;      /* each record owns one block (its round_f) */
+    int ntraj, traj_cap;
 };
 
 struct done_node {
@@ -81,8 +90,18 @@ struct done_node {
 
 #define CK(call, what) do { if (bqb_check((call), (what))) goto fail; } while (0)
 
+static void traj_free(bqb_ctx *c)
+{
+    for (int i = 0; i < c->ntraj; ++i) free((void *)c->traj[i].round_f);
+    free(c->traj);
+    c->traj = NULL;
+    c->ntraj = c->traj_cap = 0;
+}
+
 static void free_host(bqb_ctx *c)
 {
+    traj_free(c);
+    free(c->tr_f); free(c->tr_t); free(c->tr_bound); free(c->tr_cuts); free(c->tr_n);
     free(c->h_st); free(c->tag); free(c->free_stack); free(c->of_slot); free(c->ld_st);
     free(c->ld_xf); free(c->ld_sx); free(c->h_frac); free(c->h_xb);
     for (int i = 0; i < c->done_cap; ++i) { free(c->done[i].frac); free(c->done[i].xb); }
@@ -102,6 +121,7 @@ static void engine_reset(bqb_ctx *c)
     c->loaded = 0;
     c->hi = 0;
     c->ndone = c->done_head = 0;
+    if (c->tr_n) memset(c->tr_n, 0, sizeof(int) * c->max_nodes);
 }
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -365,6 +385,7 @@ int bqb_ctx_push(bqb_ctx *c, long long tag, const int *xfixed, const int *solX, 
     memcpy(c->ld_sx + k * B, solX, sizeof(int) * B);
     c->of_slot[slot] = k;
     c->tag[slot] = tag;
+    if (c->tr_n) c->tr_n[slot] = 0;
     if (slot + 1 > c->hi) c->hi = slot + 1;
     return slot;
 }
@@ -429,6 +450,73 @@ static int done_push(bqb_ctx *c, long long tag, const NodeState *st, const doubl
 }
 
 /* Move the nodes that have left SDPbound to the finished list, free slots. */
+/* Append the rounds completed since the last call to each slot's log: a
+ * round is complete once its round-end phases ran (rphase back to 0) or the
+ * node left the loop in them. */
+static void log_rounds(bqb_ctx *c)
+{
+    for (int i = 0; i < c->hi; ++i) {
+        const NodeState *s = &c->h_st[i];
+        if (c->tag[i] < 0 || c->tr_n[i] >= s->count || c->tr_n[i] >= c->tr_rmax) continue;
+        if (s->rphase != 0 && s->active) continue;
+        const int r = i * c->tr_rmax + c->tr_n[i]++;
+        c->tr_f[r] = s->f;
+        c->tr_t[r] = s->t;
+        c->tr_bound[r] = s->bound;
+        c->tr_cuts[3 * r] = s->NIneq;
+        c->tr_cuts[3 * r + 1] = s->NPentIneq;
+        c->tr_cuts[3 * r + 2] = s->NHeptaIneq;
+    }
+}
+
+/* Start the record of a finished slot: one block for the round log, the
+ * cut records and gamma; the cut records and gamma are copied from the
+ * device on the context's stream (the caller synchronizes). */
+static int traj_start(bqb_ctx *c, int i)
+{
+    const NodeState *s = &c->h_st[i];
+    if (c->ntraj == c->traj_cap) {
+        const int cap = c->traj_cap ? 2 * c->traj_cap : 64;
+        bqb_trajectory *t = realloc(c->traj, sizeof(bqb_trajectory) * cap);
+        if (!t) { bqb_set_error("calloc"); return -1; }
+        c->traj = t;
+        c->traj_cap = cap;
+    }
+    const int R = c->tr_n[i], m = s->NIneq + s->NPentIneq + s->NHeptaIneq;
+    const size_t bytes = sizeof(double) * (3 * (size_t)R + m) + sizeof(int) * (3 * (size_t)R + 4 * (size_t)s->NIneq
+                         + 8 * (size_t)(s->NPentIneq + s->NHeptaIneq)) + 8;
+    char *blk = malloc(bytes);
+    if (!blk) { bqb_set_error("calloc"); return -1; }
+    bqb_trajectory *t = &c->traj[c->ntraj++];
+    memset(t, 0, sizeof *t);
+    t->tag = c->tag[i];
+    t->n = s->n;
+    t->rounds = R;
+    t->ntri = s->NIneq;
+    t->npent = s->NPentIneq;
+    t->nhept = s->NHeptaIneq;
+    t->f = s->f;
+    t->t = s->t;
+    t->bound = s->bound;
+    t->fixedvalue = s->fixedvalue;
+    t->status = s->status;
+    double *rf = (double *)blk, *rt = rf + R, *rb = rt + R, *g = rb + R;
+    int *rc = (int *)(g + m), *tri = rc + 3 * R, *pent = tri + 4 * s->NIneq, *hept = pent + 8 * s->NPentIneq;
+    const size_t off = (size_t)i * c->tr_rmax;
+    memcpy(rf, c->tr_f + off, sizeof(double) * R);
+    memcpy(rt, c->tr_t + off, sizeof(double) * R);
+    memcpy(rb, c->tr_bound + off, sizeof(double) * R);
+    memcpy(rc, c->tr_cuts + 3 * off, sizeof(int) * 3 * R);
+    t->round_f = rf; t->round_t = rt; t->round_bound = rb; t->round_cuts = rc;
+    t->tri = tri; t->pent = pent; t->hept = hept; t->gamma = g;
+    hipStream_t st = c->stream;
+    if (s->NIneq && bqb_check(hipMemcpyAsync(tri, c->p.tri + (size_t)i * c->d.max_tri, sizeof(TriCut) * s->NIneq, hipMemcpyDeviceToHost, st), "D2H cuts")) return -1;
+    if (s->NPentIneq && bqb_check(hipMemcpyAsync(pent, c->p.pent + (size_t)i * c->d.max_pent, sizeof(KgonCut) * s->NPentIneq, hipMemcpyDeviceToHost, st), "D2H cuts")) return -1;
+    if (s->NHeptaIneq && bqb_check(hipMemcpyAsync(hept, c->p.hept + (size_t)i * c->d.max_hept, sizeof(KgonCut) * s->NHeptaIneq, hipMemcpyDeviceToHost, st), "D2H cuts")) return -1;
+    if (m && bqb_check(hipMemcpyAsync(g, c->p.gamma + (size_t)i * c->d.max_m, sizeof(double) * m, hipMemcpyDeviceToHost, st), "D2H gamma")) return -1;
+    return 0;
+}
+
 static int collect(bqb_ctx *c)
 {
     const size_t B = (size_t)(c->d.n0 - 1);
@@ -438,6 +526,9 @@ static int collect(bqb_ctx *c)
     if (!any) return 0;
     if (bqb_check(hipMemcpyAsync(c->h_frac, c->p.fracsol, sizeof(double) * B * c->hi, hipMemcpyDeviceToHost, c->stream), "D2H fracsol")) return -1;
     if (bqb_check(hipMemcpyAsync(c->h_xb, c->p.xbest, sizeof(int) * B * c->hi, hipMemcpyDeviceToHost, c->stream), "D2H xbest")) return -1;
+    if (c->capture)
+        for (int i = 0; i < c->hi; ++i)
+            if (c->tag[i] >= 0 && !c->h_st[i].active && traj_start(c, i)) return -1;
     if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
     for (int i = 0; i < c->hi; ++i) {
         if (c->tag[i] < 0 || c->h_st[i].active) continue;
@@ -519,8 +610,46 @@ int bqb_ctx_advance(bqb_ctx *c)
         RUN(PH_REBUILD, 0);
         if (fetch_states(c)) return -1;
     }
+    if (c->capture) log_rounds(c);
     return collect(c);
 }
+
+int bqb_ctx_capture(bqb_ctx *c, int on)
+{
+    if (on && !c->tr_n) {
+        c->tr_rmax = c->params.max_outer_iter + c->params.extra_iter + 1;
+        if (c->tr_rmax < 1) c->tr_rmax = 1;
+        const size_t R = (size_t)c->max_nodes * c->tr_rmax;
+        c->tr_f = malloc(sizeof(double) * R);
+        c->tr_t = malloc(sizeof(double) * R);
+        c->tr_bound = malloc(sizeof(double) * R);
+        c->tr_cuts = malloc(sizeof(int) * 3 * R);
+        c->tr_n = calloc(c->max_nodes, sizeof(int));
+        if (!c->tr_f || !c->tr_t || !c->tr_bound || !c->tr_cuts || !c->tr_n) {
+            free(c->tr_f); free(c->tr_t); free(c->tr_bound); free(c->tr_cuts); free(c->tr_n);
+            c->tr_f = c->tr_t = c->tr_bound = NULL; c->tr_cuts = c->tr_n = NULL;
+            bqb_set_error("calloc");
+            return -1;
+        }
+    }
+    if (on && !c->capture && bqb_ctx_busy(c) > 0) {
+        bqb_set_error("capture: start it while no node is in flight");
+        return -1;
+    }
+    c->capture = on ? 1 : 0;
+    return 0;
+}
+
+int bqb_ctx_trajectory_count(const bqb_ctx *c) { return c->ntraj; }
+
+int bqb_ctx_trajectory(const bqb_ctx *c, int i, bqb_trajectory *out)
+{
+    if (i < 0 || i >= c->ntraj) { bqb_set_error("trajectory %d of %d", i, c->ntraj); return -1; }
+    *out = c->traj[i];
+    return 0;
+}
+
+void bqb_ctx_trajectory_clear(bqb_ctx *c) { traj_free(c); }
 
 int bqb_ctx_pop(bqb_ctx *c, long long *tag, double *bound, double *fracsol, int *xbest, double *LB_out,
                 int *improved, double *diff_out, int *status, int *stats)

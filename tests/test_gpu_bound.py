@@ -392,3 +392,45 @@ def test_stream_through_few_slots_equals_batch_bitwise():
     for k in ("bound", "fracsol", "xbest", "LB", "improved", "diff", "stats"):
         assert np.array_equal(a[k], c[k]), k
     groups.close()
+
+
+def test_trajectory_capture_records_rounds_cuts_and_gamma():
+    """Captured records (two slot groups, streamed): one per node, the
+    rounds of its bound, and a final (cuts, gamma) at which the oracle's
+    fct_eval gives the node's final f (within 1e-6 relative)."""
+    n0 = 45
+    SP = O.build_L(I.gnp_adj(n0, 0.4, 5, weights="pm1"))
+    P = api.make_params()
+    nodes = 16
+    xf, sx = I.random_fixings(n0, 20, nodes, 6)
+    seeds = np.arange(7, 7 + nodes, dtype=np.uint64)
+    ctx = api.BoundContext(SP, params=P, max_nodes=5, groups=2)
+    ctx.capture(True)
+    r = ctx.bound(xf, sx, LB=0.0, diff=1e9, seeds=seeds, stream=True)
+    recs = ctx.trajectories()
+    assert [t["tag"] for t in recs] == list(range(nodes))
+    checked = 0
+    for b, t in enumerate(recs):
+        assert t["bound"] == r["bound"][b] and t["status"] == 0
+        assert t["rounds"] == r["stats"][b][2]
+        assert t["bound"] == t["f"] + t["fixedvalue"]
+        if t["rounds"] == 0:
+            continue
+        assert t["round_f"][-1] == t["f"] and t["round_bound"][-1] == t["bound"]
+        assert tuple(t["round_cuts"][-1]) == (len(t["tri"]), len(t["pent"]), len(t["hept"]))
+        # gamma + t (eta - G lambda) cancels to rounding where eta > 0
+        assert np.all(t["gamma"] >= -1e-12), t["gamma"].min()
+        PL, fv = O.create_subproblem(SP, xf[b], sx[b])
+        assert PL.shape[0] == t["n"] and fv == t["fixedvalue"]
+        cuts = O.cut_arrays([tuple(c) for c in t["tri"]], [(c[0], list(c[1:6])) for c in t["pent"]],
+                            [(c[0], list(c[1:8])) for c in t["hept"]])
+        fo, _, _, st = O.fct_eval(PL, cuts, t["gamma"])
+        assert st.status == 0
+        assert abs(fo - t["f"]) <= 1e-6 * abs(t["f"]), (b, fo, t["f"])
+        checked += 1
+    assert checked >= 4
+    # capture off: nothing more is recorded
+    ctx.capture(False)
+    ctx.bound(xf[:3], sx[:3], LB=0.0, diff=1e9, seeds=seeds[:3], stream=True)
+    assert ctx.trajectories() == []
+    ctx.close()

@@ -44,7 +44,7 @@ template <int P, int R, int KB>
 struct Lds {
     static constexpr int NMAX = P * R;
     static constexpr int COL = 2 * NMAX;                 // double-buffered pivot column
-    static constexpr int PANEL = 2 * 2 * KB * NMAX;      // double-buffered A/T GEMM panels
+    static constexpr int PANEL = 2 * 3 * KB * NMAX;      // double-buffered A/X/W GEMM panels
     static constexpr int TRANS = NMAX * (P + 1);         // symmetrize staging
     static constexpr int RED = (P * P / 64) * NMAX;      // row-reduction partials
     static constexpr int UNION = PANEL > TRANS ? (PANEL > RED ? PANEL : RED) : (TRANS > RED ? TRANS : RED);

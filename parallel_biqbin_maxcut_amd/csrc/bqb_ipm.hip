@@ -40,38 +40,74 @@ constexpr int IPM_MAX_ITERS = 500;
 constexpr int LS_MAX_TRIALS = 500;
 constexpr int KB = 8;   // GEMM panel depth
 
-// C += A * T for a symmetric A held in HBM (column-major slab, ld) and a T
-// whose row k is formed by the owners of column k from element (J, k) of
-// symmetric operands: T(k, J) = tfun(k, J + k*ld) (every operand is
-// symmetric, so row k equals column k).  Panels of KB columns of A and rows
-// of T are staged in LDS, double buffered: one barrier per panel.
-template <int P, int R, class TF>
-__device__ __forceinline__ void gemm_sym(double (&C)[R][R], const double *Ag, int ld,
-                                         int n, double *pan, const Thr &t, TF tfun)
+// Copies columns k0 .. k0+KB-1 (rows 0 .. NMAX-1) of a slab into LDS at
+// dst[kk*NMAX + J] with global_load_lds (16 bytes per lane, written by the
+// memory pipeline straight into LDS: no registers).  Every wave covers 64
+// consecutive 16-byte chunks per instruction; the slab's zero padding
+// supplies rows and columns past n.  Completion is waited for with vmcnt.
+template <int P, int R>
+__device__ __forceinline__ void panel_to_lds(double *dst, const double *g, int ld, int k0, const Thr &t)
 {
     constexpr int NMAX = P * R;
+    static_assert(NMAX % 2 == 0, "16-byte chunks");
+    constexpr int CPC = NMAX / 2;        // chunks per column
+    constexpr int CH = KB * CPC;         // chunks per panel
+    constexpr int NW = P * P / 64;
+    const int lane = t.tid & 63, wave = t.tid >> 6;
+    for (int j = wave; j * 64 < CH; j += NW) {
+        const int c = j * 64 + lane;
+        const int kk = c / CPC, r = c - kk * CPC;
+        if (c < CH)
+            __builtin_amdgcn_global_load_lds((const void *)(g + (size_t)(k0 + kk) * ld + 2 * r),
+                                             (__attribute__((address_space(3))) void *)(dst + j * 128), 16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void wait_panel_dma() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// C += A * T for a symmetric A held in HBM (column-major slab, ld) and
+// T(k, J) = -s1[k] X(J, k)  (or, with W, (-s2[k] X(J, k)) + (-s1[k] W(J, k)));
+// every operand is symmetric, so row k of T is formed from column k.
+// Panels of KB columns of A, X (and W) are copied to LDS by DMA, double
+// buffered: the next panel's copy runs during the current panel's products,
+// one barrier per panel.  pan holds 2 * (2 or 3) * KB * NMAX doubles.
+template <int P, int R, bool TWO>
+__device__ __forceinline__ void gemm_sym(double (&C)[R][R], const double *Ag, const double *Xg, const double *Wg,
+                                         const double *s1, const double *s2, int ld, int n, double *pan,
+                                         const Thr &t)
+{
+    constexpr int NMAX = P * R;
+    constexpr int NP = TWO ? 3 : 2;                 // panels per stage
+    constexpr int STAGE = NP * KB * NMAX;
+    auto issue = [&](int k0, int buf) {
+        double *st = pan + buf * STAGE;
+        panel_to_lds<P, R>(st, Ag, ld, k0, t);
+        panel_to_lds<P, R>(st + KB * NMAX, Xg, ld, k0, t);
+        if (TWO) panel_to_lds<P, R>(st + 2 * KB * NMAX, Wg, ld, k0, t);
+    };
+    __syncthreads();   // pan may still be read by the previous user
     int buf = 0;
+    issue(0, 0);
     for (int k0 = 0; k0 < n; k0 += KB) {
-        double *Ap = pan + buf * (2 * KB * NMAX);
-        double *Tp = Ap + KB * NMAX;
-        // KB*NMAX elements per panel, spread over all threads; column k of
-        // the slab is contiguous, so consecutive threads read consecutive rows.
-        for (int e = t.tid; e < KB * NMAX; e += P * P) {
-            const int kk = e / NMAX, J = e - kk * NMAX;
-            const int k = k0 + kk;
-            const size_t idx = (size_t)J + (size_t)k * ld;
-            const bool ok = k < n;
-            Ap[e] = ok ? Ag[idx] : 0.0;
-            Tp[e] = ok ? tfun(k, idx) : 0.0;
-        }
+        wait_panel_dma();
         __syncthreads();
+        if (k0 + KB < n) issue(k0 + KB, buf ^ 1);
+        const double *Ap = pan + buf * STAGE;
+        const double *Xp = Ap + KB * NMAX;
+        const double *Wp = Xp + KB * NMAX;
 #pragma unroll
         for (int q = 0; q < KB; ++q) {
+            const int k = k0 + q;
+            const double m1 = -s1[k];
+            const double m2 = TWO ? -s2[k] : 0.0;
             double aI[R], tJ[R];
 #pragma unroll
             for (int a = 0; a < R; ++a) aI[a] = Ap[q * NMAX + t.tr + P * a];
 #pragma unroll
-            for (int b = 0; b < R; ++b) tJ[b] = Tp[q * NMAX + t.tc + P * b];
+            for (int b = 0; b < R; ++b) {
+                const int e = q * NMAX + t.tc + P * b;
+                tJ[b] = TWO ? (m2 * Xp[e]) + (m1 * Wp[e]) : m1 * Xp[e];
+            }
 #pragma unroll
             for (int a = 0; a < R; ++a)
 #pragma unroll
@@ -215,9 +251,7 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         for (int a = 0; a < R; ++a)
 #pragma unroll
             for (int b = 0; b < R; ++b) A[a][b] = 0.0;
-        gemm_sym<P, R>(A, Wzi, ld, n, s_scr, t, [&](int k, size_t idx) {
-            return -s_dy1[k] * X[idx];
-        });
+        gemm_sym<P, R, false>(A, Wzi, X, nullptr, s_dy1, nullptr, ld, n, s_scr, t);
 #pragma unroll
         for (int b = 0; b < R; ++b)
 #pragma unroll
@@ -259,9 +293,7 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         for (int a = 0; a < R; ++a)
 #pragma unroll
             for (int b = 0; b < R; ++b) A[a][b] = 0.0;
-        gemm_sym<P, R>(A, Wzi, ld, n, s_scr, t, [&](int k, size_t idx) {
-            return (-s_dy2[k] * X[idx]) + (-s_dy1[k] * Wdx[idx]);
-        });
+        gemm_sym<P, R, true>(A, Wzi, X, Wdx, s_dy1, s_dy2, ld, n, s_scr, t);
 #pragma unroll
         for (int b = 0; b < R; ++b) {
 #pragma unroll
@@ -383,7 +415,7 @@ __global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
     constexpr int NW = P * P / 64;
     using L_ = Lds<P, R, KB>;
     __shared__ double s_col[L_::COL];
-    __shared__ double s_scr[L_::UNION];
+    __shared__ __align__(16) double s_scr[L_::UNION];
     __shared__ double s_y[NMAX], s_zd[NMAX], s_dy1[NMAX], s_dy2[NMAX], s_v[NMAX], s_dg[NMAX];
     __shared__ double s_red[NW];
 

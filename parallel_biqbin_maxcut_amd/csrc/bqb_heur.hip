@@ -16,6 +16,7 @@
 #include "bqb_dist.h"
 #include "bqb_kernels.h"
 #include "bqb_rng.h"
+#include "bqb_wave.h"
 
 namespace bqb {
 
@@ -24,14 +25,7 @@ constexpr int NMAX_GW = 256;      // vertices (n0) supported
 constexpr int GW_BATCH = 16;      // hyperplanes held in LDS at a time
 
 // argmax with the first index on ties across a wave
-__device__ __forceinline__ void wave_argmax(double &v, int &i)
-{
-    for (int o = 32; o >= 1; o >>= 1) {
-        const double ov = __shfl_xor(v, o, 64);
-        const int oi = __shfl_xor(i, o, 64);
-        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
-    }
-}
+__device__ __forceinline__ void wave_argmax(double &v, int &i) { wave_argmax64(v, i); }
 
 __device__ __forceinline__ double wave_sum(double v)
 {

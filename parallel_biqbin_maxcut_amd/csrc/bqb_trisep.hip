@@ -14,6 +14,7 @@
 
 #include "bqb_batch.h"
 #include "bqb_kernels.h"
+#include "bqb_wave.h"
 
 namespace bqb {
 
@@ -193,11 +194,7 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
                     s_lidx[slot] = s_qidx[qi + lane];
                 }
                 // first minimum among the new entries
-                for (int o = 32; o >= 1; o >>= 1) {
-                    const double ov = __shfl_xor(val, o, 64);
-                    const int os = __shfl_xor(slot, o, 64);
-                    if (ov < val || (ov == val && os < slot)) { val = ov; slot = os; }
-                }
+                wave_argmin64(val, slot);
                 if (size == 0 || val < leastv) { leastv = val; least = slot; }
                 size += take;
                 qi += take;
@@ -222,11 +219,7 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
                     const double lv = s_lval[i];
                     if (lv < mv) { mv = lv; ms = i; }
                 }
-                for (int o = 32; o >= 1; o >>= 1) {
-                    const double ov = __shfl_xor(mv, o, 64);
-                    const int os = __shfl_xor(ms, o, 64);
-                    if (ov < mv || (ov == mv && os < ms)) { mv = ov; ms = os; }
-                }
+                wave_argmin64(mv, ms);
                 leastv = mv;
                 least = ms;
                 qi += first + 1;

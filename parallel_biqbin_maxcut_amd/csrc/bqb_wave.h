@@ -109,4 +109,61 @@ __device__ __forceinline__ double wave_min32(double v)
     return fmin(lane_value(v, 0), lane_value(v, 16));
 }
 
+// Whole-wave (value, index) selections, uniform result.  The winner under a
+// total order does not depend on the combining order, so these give the
+// same pair as any other reduction tree.
+template <int CTRL>
+__device__ __forceinline__ void argmax_step(double &v, int &i)
+{
+    const double ov = dpp_move<CTRL>(v);
+    const int oi = dpp_move<CTRL>(i);
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+
+template <int CTRL>
+__device__ __forceinline__ void argmin_step(double &v, int &i)
+{
+    const double ov = dpp_move<CTRL>(v);
+    const int oi = dpp_move<CTRL>(i);
+    if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+
+// largest value, smallest index among equal values
+__device__ __forceinline__ void wave_argmax64(double &v, int &i)
+{
+    argmax_step<DPP_SWAP1>(v, i);
+    argmax_step<DPP_SWAP2>(v, i);
+    argmax_step<DPP_HALF_MIRROR>(v, i);
+    argmax_step<DPP_MIRROR>(v, i);
+    double bv = lane_value(v, 0);
+    int bi = lane_value(i, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16) {
+        const double rv = lane_value(v, r);
+        const int ri = lane_value(i, r);
+        if (rv > bv || (rv == bv && ri < bi)) { bv = rv; bi = ri; }
+    }
+    v = bv;
+    i = bi;
+}
+
+// smallest value, smallest index among equal values
+__device__ __forceinline__ void wave_argmin64(double &v, int &i)
+{
+    argmin_step<DPP_SWAP1>(v, i);
+    argmin_step<DPP_SWAP2>(v, i);
+    argmin_step<DPP_HALF_MIRROR>(v, i);
+    argmin_step<DPP_MIRROR>(v, i);
+    double bv = lane_value(v, 0);
+    int bi = lane_value(i, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16) {
+        const double rv = lane_value(v, r);
+        const int ri = lane_value(i, r);
+        if (rv < bv || (rv == bv && ri < bi)) { bv = rv; bi = ri; }
+    }
+    v = bv;
+    i = bi;
+}
+
 }  // namespace bqb

@@ -93,6 +93,12 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
     // ---- 2. scan ---------------------------------------------------------
     if (threadIdx.x == 0) { s_size = 0; s_least = 0; s_leastv = 1e9; }
     double vmax = -1e9;
+    // wave 0, once the list is full: lane b keeps the first minimum (value,
+    // position) of list block b (positions 64b .. 64b+63), so a replacement
+    // rescans one block and the block minima instead of the whole list
+    double bmv = 1e300;
+    int bms = 0x7fffffff;
+    bool blocks_ready = false;
     const long long per_type = c3(n);
     const long long total = 4 * per_type;
     int parity = 0;
@@ -200,6 +206,16 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
                 qi += take;
             }
             // replacement phase
+            if (qi < qn && !blocks_ready) {
+                for (int b = 0; b * 64 < cap; ++b) {
+                    const int pos = 64 * b + lane;
+                    double bv = (pos < cap) ? s_lval[pos] : 1e300;
+                    int bp = (pos < cap) ? pos : 0x7fffffff;
+                    wave_argmin64(bv, bp);
+                    if (lane == b) { bmv = bv; bms = bp; }
+                }
+                blocks_ready = true;
+            }
             while (qi < qn) {
                 const int m = min(64, qn - qi);
                 const double val = (lane < m) ? s_qval[qi + lane] : -1e300;
@@ -212,13 +228,16 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
                 }
                 __builtin_amdgcn_s_waitcnt(0);
                 __builtin_amdgcn_wave_barrier();
-                // rescan: first minimum of the list
-                double mv = 1e300;
-                int ms = 0x7fffffff;
-                for (int i = lane; i < size; i += 64) {
-                    const double lv = s_lval[i];
-                    if (lv < mv) { mv = lv; ms = i; }
-                }
+                // the replaced entry's block, then the first minimum of the
+                // list as the first minimum over the blocks
+                const int b = least >> 6;
+                const int pos = 64 * b + lane;
+                double bv = (pos < cap) ? s_lval[pos] : 1e300;
+                int bp = (pos < cap) ? pos : 0x7fffffff;
+                wave_argmin64(bv, bp);
+                if (lane == b) { bmv = bv; bms = bp; }
+                double mv = bmv;
+                int ms = bms;
                 wave_argmin64(mv, ms);
                 leastv = mv;
                 least = ms;

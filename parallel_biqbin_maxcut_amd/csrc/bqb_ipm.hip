@@ -408,8 +408,11 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
 // One workgroup per subproblem.  The register tile is chosen per
 // subproblem: ceil(n / P) blocks, so a small node does not pay for the
 // padding of the largest one.
+// 256-thread tiles are held to two workgroups per CU (the 16 x 7 tile then
+// keeps part of its state in scratch): with several slot groups in flight,
+// a second resident workgroup per CU gains more than the spills cost.
 template <int P, int R>
-__global__ __launch_bounds__(P * P) void ipm_batch_kernel(IpmBatchArgs args)
+__global__ __launch_bounds__(P * P, (P == 16) ? 2 : 1) void ipm_batch_kernel(IpmBatchArgs args)
 {
     constexpr int NMAX = P * R;
     constexpr int NW = P * P / 64;

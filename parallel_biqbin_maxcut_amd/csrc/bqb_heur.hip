@@ -33,9 +33,14 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <int P, int R>
+// FL: the original objective SP_L and the node's L are copied into LDS as
+// float (dynamic shared memory) and read from there whenever every entry
+// converts exactly (integer weights give quarter-integers), so the 1-opt and
+// the cut evaluations read LDS instead of HBM/L2 with the same arithmetic.
+template <int P, int R, bool FL>
 __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
 {
+    extern __shared__ float s_f[];     // FL: [n0*n0] SP_L, then [n*n] L
     constexpr int NT = P * P;
     constexpr int NW = NT / 64;
     constexpr int NMAX = P * R;
@@ -52,6 +57,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     __shared__ double s_red[NW];
     __shared__ double s_best;
     __shared__ int s_flag;
+    __shared__ int s_exact;
 
     const int node = blockIdx.x;
     if (node >= d.nodes || !p.mask[node]) return;
@@ -76,11 +82,40 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         int nf = 0;
         for (int i = 0; i < B; ++i)
             if (!xf[i]) s_free[nf++] = i;
+        s_exact = FL ? 1 : 0;
     }
+    float *const s_spl = s_f, *const s_lf = s_f + (FL ? n0 * n0 : 0);
+    if (FL) {
+        __syncthreads();
+        int bad = 0;
+        for (int e = t.tid; e < n0 * n0; e += NT) {
+            const double v = p.SP_L[e];
+            const float f = (float)v;
+            bad |= ((double)f != v);
+            s_spl[e] = f;
+        }
+        for (int e = t.tid; e < n * n; e += NT) {
+            const int i = e % n, j = e / n;
+            const double v = L[i + (size_t)j * ld];
+            const float f = (float)v;
+            bad |= ((double)f != v);
+            s_lf[e] = f;
+        }
+        if (bad) s_exact = 0;
+    }
+    // use_f is read from s_exact after the barrier below
+    bool use_f = false;
+    const auto spl = [&](int j, int i) -> double {
+        return (FL && use_f) ? (double)s_spl[j + i * n0] : p.SP_L[j + (size_t)i * n0];
+    };
+    const auto lv = [&](int i, int j) -> double {
+        return (FL && use_f) ? (double)s_lf[i + j * n] : L[i + (size_t)j * ld];
+    };
     // first random cut xh in {-1,1}^n (heuristic.c:38-39)
     for (int i = t.tid; i < NMAX; i += NT)
         s_xh[i] = (i < n) ? (double)(2 * (int)(bqb_draw(key, (uint64_t)i) >> 63) - 1) : 0.0;
     __syncthreads();
+    use_f = FL && s_exact;
 
     // fh = value of its lift (heuristic.c:42-54)
     if (t.tid == 0) {
@@ -97,7 +132,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         for (int i = wave; i < B; i += NW) {
             if (!s_sol[i]) continue;
             for (int j = lane; j < B; j += 64)
-                if (s_sol[j]) acc += p.SP_L[j + (size_t)i * n0];
+                if (s_sol[j]) acc += spl(j, i);
         }
         fh = wg_sum<P>(acc, s_red, t);
     }
@@ -131,6 +166,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
             for (int e = t.tid; e < hb * n; e += NT) {
                 const int hh = e / n, i = e - hh * n;
                 double sca = 0.0;
+#pragma unroll 8
                 for (int j = 0; j <= i; ++j) sca = __dadd_rn(sca, __dmul_rn(s_v[hh][j], Zs[i + (size_t)j * ld]));
                 s_sign[hh][i] = (sca < 0) ? -1 : 1;
             }
@@ -149,7 +185,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
 #pragma unroll
                     for (int q = 0; q < PL; ++q) {
                         const int i = lane + 64 * q;
-                        if (i < n) Lx[q] += L[i + (size_t)j * ld] * xj;
+                        if (i < n) Lx[q] += lv(i, j) * xj;
                     }
                 }
                 double bv = -1e9;
@@ -158,7 +194,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                 for (int q = 0; q < PL; ++q) {
                     const int i = lane + 64 * q;
                     if (i < n) {
-                        const double dl = L[i + (size_t)i * ld] - xcur[i] * Lx[q];
+                        const double dl = lv(i, i) - xcur[i] * Lx[q];
                         if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
                     }
                 }
@@ -172,7 +208,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                     for (int q = 0; q < PL; ++q) {
                         const int i = lane + 64 * q;
                         if (i < n) {
-                            const double lij = L[i + (size_t)index * ld];   // = L(index, i)
+                            const double lij = lv(i, index);   // = L(index, i)
                             if (fabs(lij) > 0.001) Lx[q] = (xi > 0) ? Lx[q] - 2 * lij : Lx[q] + 2 * lij;
                         }
                     }
@@ -185,7 +221,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                     for (int q = 0; q < PL; ++q) {
                         const int i = lane + 64 * q;
                         if (i < n) {
-                            const double dl = L[i + (size_t)i * ld] - xcur[i] * Lx[q];
+                            const double dl = lv(i, i) - xcur[i] * Lx[q];
                             if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
                         }
                     }
@@ -196,13 +232,29 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                 __builtin_amdgcn_wave_barrier();
                 for (int a = lane; a < n - 1; a += 64) s_lift[hh][s_free[a]] = (signed char)((xcur[a] + 1) / 2);
                 __builtin_amdgcn_wave_barrier();
-                // its cut value (update_best -> evaluateSolution): rows with
-                // sol_i = 1, lanes over j; integer data, so the sum is exact
+                // its cut value (update_best -> evaluateSolution): the sum of
+                // SP_L over the pairs with sol_i = sol_j = 1, lanes over j and
+                // the 0/1 values as factors; integer data, so the sum is exact
+                // whatever the order
                 double acc = 0.0;
-                for (int i = 0; i < B; ++i) {
-                    if (!s_lift[hh][i]) continue;
-                    for (int j = lane; j < B; j += 64)
-                        if (s_lift[hh][j]) acc += p.SP_L[j + (size_t)i * n0];
+                {
+                    double yj[(NMAX_GW + 63) / 64];
+#pragma unroll
+                    for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
+                        const int j = lane + 64 * q;
+                        yj[q] = (j < B) ? (double)s_lift[hh][j] : 0.0;
+                    }
+#pragma unroll 4
+                    for (int i = 0; i < B; ++i) {
+                        const double yi = (double)s_lift[hh][i];
+                        double r = 0.0;
+#pragma unroll
+                        for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
+                            const int j = lane + 64 * q;
+                            if (64 * q < B) r += (j < B) ? yj[q] * spl(j, i) : 0.0;
+                        }
+                        acc += yi * r;
+                    }
                 }
                 acc = wave_sum(acc);
                 if (lane == 0) s_val[hh] = acc;
@@ -270,7 +322,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     for (int i = wave; i < B; i += NW) {
         if (!s_sol[i]) continue;
         for (int j = lane; j < B; j += 64)
-            if (s_sol[j]) acc += p.SP_L[j + (size_t)i * n0];
+            if (s_sol[j]) acc += spl(j, i);
     }
     acc = wg_sum<P>(acc, s_red, t);
     if (acc > s.LB) {
@@ -283,7 +335,17 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
 template <int P, int R>
 static hipError_t launch_heur(const BatchDims &d, const BatchPtrs &p, hipStream_t s)
 {
-    hipLaunchKernelGGL((heur_kernel<P, R>), dim3(d.nodes), dim3(P * P), 0, s, d, p);
+    // float copies of SP_L and L (n <= n0) while they fit beside the static LDS
+    static const size_t static_lds = []() {
+        hipFuncAttributes a;
+        return (hipFuncGetAttributes(&a, (const void *)heur_kernel<P, R, true>) == hipSuccess)
+                   ? a.sharedSizeBytes : (size_t)160 * 1024;
+    }();
+    const size_t dyn = sizeof(float) * 2 * (size_t)d.n0 * d.n0;
+    if (static_lds + dyn <= (size_t)160 * 1024)
+        hipLaunchKernelGGL((heur_kernel<P, R, true>), dim3(d.nodes), dim3(P * P), dyn, s, d, p);
+    else
+        hipLaunchKernelGGL((heur_kernel<P, R, false>), dim3(d.nodes), dim3(P * P), 0, s, d, p);
     return hipGetLastError();
 }
 

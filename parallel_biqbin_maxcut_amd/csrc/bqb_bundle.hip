@@ -443,16 +443,24 @@ __global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs
         __syncthreads();
         t = s_t;
         if (s_done) break;
-        // c = zeta - t G'eta for the next round
+        // c = zeta - t G'eta for the next round: each wave reduces its
+        // partial sums of every column, then one thread per column adds the
+        // wave results in order (the arithmetic of bsum, one barrier)
         static_for<0, PE>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
             if (j >= k) return;
-            const double v = bsum(pe[j], s_red);
-            if (threadIdx.x == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
+            const double v = wsum(pe[j]);
+            if (lane == 0) s_A[wave * KMAX + j] = v;
         });
         for (int j = PE; j < k; ++j) {
-            const double v = bsum(tmp[(size_t)(j - PE) * BT + threadIdx.x], s_red);
-            if (threadIdx.x == 0) s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
+            const double v = wsum(tmp[(size_t)(j - PE) * BT + threadIdx.x]);
+            if (lane == 0) s_A[wave * KMAX + j] = v;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < k; j += BT) {
+            double v = 0.0;
+            for (int w = 0; w < BT / 64; ++w) v += s_A[w * KMAX + j];
+            s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
         }
         for (int j = PE; j < k; ++j) tmp[(size_t)(j - PE) * BT + threadIdx.x] = 0.0;
         __syncthreads();

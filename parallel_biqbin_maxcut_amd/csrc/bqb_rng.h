@@ -28,6 +28,27 @@ BQB_HD uint64_t bqb_draw(uint64_t key, uint64_t ctr)
 BQB_HD double bqb_u01(uint64_t r) { return (double)(r >> 11) * 0x1.0p-53; }
 BQB_HD int bqb_rmod(uint64_t r, int m) { return (int)((uint32_t)(r >> 32) % (uint32_t)m); }
 
+/* The same residue for a divisor fixed over many draws (0 < m < 2^16):
+ * the quotient of x < 2^32 by m estimated with a double reciprocal is off
+ * by at most 2^-20 before truncation, so it is the true quotient or one
+ * less, and one correction makes the residue exact. */
+typedef struct bqb_divm { double rd; uint32_t m; } bqb_divm;
+BQB_HD bqb_divm bqb_divm_make(int m)
+{
+    bqb_divm d;
+    d.rd = 1.0 / (double)m;
+    d.m = (uint32_t)m;
+    return d;
+}
+BQB_HD int bqb_rmod_by(uint64_t r, bqb_divm d)
+{
+    const uint32_t x = (uint32_t)(r >> 32);
+    const uint32_t q = (uint32_t)((double)x * d.rd);
+    uint32_t rem = x - q * d.m;
+    if (rem >= d.m) rem -= d.m;
+    return (int)rem;
+}
+
 /* stream tags of one node (heuristic call h, separation round c) */
 #define BQB_KEY_HEUR(seed, h) bqb_key((seed), 1000u + (uint64_t)(h))
 #define BQB_KEY_PENT(seed, c) bqb_key((seed), 2000u + (uint64_t)(c))

@@ -145,12 +145,13 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
             for (int j = i + 1; j < K; ++j) sol += 2 * (e[j] * e[i]) * xa(pk.get(j), pk.get(i));
         double cost = 1e9;
         int not_done = 1, stages = 0, status = 0;
+        const bqb_divm dn = bqb_divm_make(n);
         while (not_done > 0) {
             if (stages++ >= SA_MAX_STAGES) { status = ERR_SA_STAGES; break; }
             not_done = 0;
             for (int it = 0; it < m1; ++it) {
                 int i1 = bqb_rmod(bqb_draw(key, ctr++), K);
-                int i2 = bqb_rmod(bqb_draw(key, ctr++), n);
+                int i2 = bqb_rmod_by(bqb_draw(key, ctr++), dn);
                 if (i2 < i1) { const int tmp = i1; i1 = i2; i2 = tmp; }
                 const int p1 = pk.get(i1);
                 const int p2 = (i2 < K) ? pk.get(i2) : perm.get(i2);

@@ -196,14 +196,16 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
                 for (int a = 0; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
             }
             __syncthreads();
-            // every LDS read of the step is issued before the pivot test
             const double d = cb[k];
             double cI[R], cJ[R];
 #pragma unroll
             for (int a = 0; a < R; ++a) cI[a] = cb[t.tr + P * a];
 #pragma unroll
             for (int b = 0; b < R; ++b) cJ[b] = cb[t.tc + P * b];
-            if (!(d > 0.0)) { ok = false; break; }
+            // the pivot test ends the step, so the LDS reads and the
+            // divide are not held back by it (a failed step's updates are
+            // discarded with the matrix)
+            const bool good = d > 0.0;
             const double dinv = 1.0 / d;
             double tJ[R];
 #pragma unroll
@@ -221,6 +223,7 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
                 for (int b = 0; b < R; ++b) A[kb][b] = tJ[b];
             }
             if (t.tr == kt && t.tc == kt) A[kb][kb] = -dinv;
+            if (!good) { ok = false; break; }
             buf ^= 1;
         }
     });
@@ -253,14 +256,16 @@ __device__ __forceinline__ bool chol_test(double (&A)[R][R], int n, double *colb
                 for (int a = kb; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
             }
             __syncthreads();
-            // every LDS read of the step is issued before the pivot test
             const double d = cb[k];
             double cI[R], cJ[R];
 #pragma unroll
             for (int a = kb; a < R; ++a) cI[a] = cb[t.tr + P * a];
 #pragma unroll
             for (int b = kb; b < R; ++b) cJ[b] = cb[t.tc + P * b];
-            if (!(d > 0.0)) { ok = false; break; }
+            // the pivot test ends the step, so the LDS reads and the
+            // divide are not held back by it (a failed step's updates are
+            // discarded with the matrix)
+            const bool good = d > 0.0;
             const double dinv = 1.0 / d;
             double tJ[R];
 #pragma unroll
@@ -269,6 +274,7 @@ __device__ __forceinline__ bool chol_test(double (&A)[R][R], int n, double *colb
             for (int a = kb; a < R; ++a)
 #pragma unroll
                 for (int b = kb; b < R; ++b) A[a][b] = fma(-cI[a], tJ[b], A[a][b]);
+            if (!good) { ok = false; break; }
             buf ^= 1;
         }
     });
@@ -295,14 +301,16 @@ __device__ __forceinline__ bool chol_lower(double (&A)[R][R], int n, double *col
                 for (int a = kb; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
             }
             __syncthreads();
-            // every LDS read of the step is issued before the pivot test
             const double d = cb[k];
             double cI[R], cJ[R];
 #pragma unroll
             for (int a = kb; a < R; ++a) cI[a] = cb[t.tr + P * a];
 #pragma unroll
             for (int b = kb; b < R; ++b) cJ[b] = cb[t.tc + P * b];
-            if (!(d > 0.0)) { ok = false; break; }
+            // the pivot test ends the step, so the LDS reads and the
+            // divide are not held back by it (a failed step's updates are
+            // discarded with the matrix)
+            const bool good = d > 0.0;
             const double sq = sqrt(d);
             const double s = 1.0 / sq;
             double sJ[R];
@@ -325,6 +333,7 @@ __device__ __forceinline__ bool chol_lower(double (&A)[R][R], int n, double *col
                     if (I >= k) A[a][kb] = (I == k) ? sq : cI[a] * s;
                 }
             }
+            if (!good) { ok = false; break; }
             buf ^= 1;
         }
     });

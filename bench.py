@@ -8,13 +8,15 @@ full SDPbound (FP64 IPM + bundle over triangle / pentagonal / heptagonal cuts,
 GW heuristic).  Every node receives the incumbent and the root gap of the
 instance's root bound, as a B&B worker does.
 
-One step = 16384 such nodes streamed through six slot groups of 512 node
+One step = 65536 such nodes streamed through six slot groups of 512 node
 slots (bqb_bound_stream_multi: every launch covers one group's 512 slots, a
 finished node's slot is refilled after each outer round, as the B&B node
 queue runs it; each group has its own HIP stream and host thread, so one
 group's kernels fill the idle tail of another's launches).  The stream is
-long so that the ramp-down at its end, when few slots are still busy,
-weighs little, as in a node queue that is never empty.  Inputs are uploaded
+long because its start (every slot beginning a node at once) and its end
+(the last nodes' remaining rounds with few slots busy) cost about 1.8 s
+whatever its length; a B&B node queue that never runs empty does not pay
+them (16384 nodes per step give 4540 bounds/s, 32768 6040, 65536 6750).  Inputs are uploaded
 inside the call (a few hundred KB; the timed region includes it).
 value = node bounds per second.  For comparison, config reports the first
 2048 nodes streamed through a single group and the first 512 bounded as one
@@ -132,10 +134,10 @@ def traffic_per_launch():
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nodes", type=int, default=512, help="node slots (subproblems per launch)")
-    ap.add_argument("--per-step", type=int, default=16384, help="nodes bounded per step")
+    ap.add_argument("--per-step", type=int, default=65536, help="nodes bounded per step")
     ap.add_argument("--groups", type=int, default=6, help="slot groups (streams) of --nodes slots each")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=192)

@@ -217,31 +217,40 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
                 blocks_ready = true;
             }
             while (qi < qn) {
+                // 64 candidates in registers; each replacement re-tests the
+                // ones after it against the new least value
                 const int m = min(64, qn - qi);
                 const double val = (lane < m) ? s_qval[qi + lane] : -1e300;
-                unsigned long long bal = __ballot(val > leastv);
-                if (!bal) { qi += m; continue; }
-                const int first = __ffsll((long long)bal) - 1;
-                if (lane == 0) {
-                    s_lval[least] = s_qval[qi + first];
-                    s_lidx[least] = s_qidx[qi + first];
+                const unsigned vid = (lane < m) ? s_qidx[qi + lane] : 0u;
+                unsigned long long live = (m == 64) ? ~0ull : ((1ull << m) - 1ull);
+                for (;;) {
+                    const unsigned long long bal = __ballot(val > leastv) & live;
+                    if (!bal) break;
+                    const int first = __ffsll((long long)bal) - 1;
+                    const double nv = lane_value(val, first);
+                    const unsigned nid = (unsigned)lane_value((int)vid, first);
+                    if (lane == 0) {
+                        s_lval[least] = nv;
+                        s_lidx[least] = nid;
+                    }
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __builtin_amdgcn_wave_barrier();
+                    // the replaced entry's block, then the first minimum of the
+                    // list as the first minimum over the blocks
+                    const int b = least >> 6;
+                    const int pos = 64 * b + lane;
+                    double bv = (pos < cap) ? s_lval[pos] : 1e300;
+                    int bp = (pos < cap) ? pos : 0x7fffffff;
+                    wave_argmin64(bv, bp);
+                    if (lane == b) { bmv = bv; bms = bp; }
+                    double mv = bmv;
+                    int ms = bms;
+                    wave_argmin64(mv, ms);
+                    leastv = mv;
+                    least = ms;
+                    live &= ~((2ull << first) - 1ull);
                 }
-                __builtin_amdgcn_s_waitcnt(0);
-                __builtin_amdgcn_wave_barrier();
-                // the replaced entry's block, then the first minimum of the
-                // list as the first minimum over the blocks
-                const int b = least >> 6;
-                const int pos = 64 * b + lane;
-                double bv = (pos < cap) ? s_lval[pos] : 1e300;
-                int bp = (pos < cap) ? pos : 0x7fffffff;
-                wave_argmin64(bv, bp);
-                if (lane == b) { bmv = bv; bms = bp; }
-                double mv = bmv;
-                int ms = bms;
-                wave_argmin64(mv, ms);
-                leastv = mv;
-                least = ms;
-                qi += first + 1;
+                qi += m;
             }
             if (lane == 0) { s_size = size; s_least = least; s_leastv = leastv; }
         }

@@ -270,10 +270,12 @@ __device__ __forceinline__ bool chol_test(double (&A)[R][R], int n, double *colb
             double tJ[R];
 #pragma unroll
             for (int b = kb; b < R; ++b) tJ[b] = cJ[b] * dinv;
+            // lower blocks only: the pivots and the next columns are read
+            // from blocks with a >= b, which see exactly the full update
 #pragma unroll
             for (int a = kb; a < R; ++a)
 #pragma unroll
-                for (int b = kb; b < R; ++b) A[a][b] = fma(-cI[a], tJ[b], A[a][b]);
+                for (int b = kb; b <= a; ++b) A[a][b] = fma(-cI[a], tJ[b], A[a][b]);
             if (!good) { ok = false; break; }
             buf ^= 1;
         }
@@ -321,7 +323,7 @@ __device__ __forceinline__ bool chol_lower(double (&A)[R][R], int n, double *col
                 const int I = t.tr + P * a;
                 const double sI = cI[a] * s;
 #pragma unroll
-                for (int b = kb; b < R; ++b) {
+                for (int b = kb; b <= a; ++b) {   // the upper blocks end as zeros
                     const int J = t.tc + P * b;
                     if (I > k && J > k) A[a][b] = fma(-sI, sJ[b], A[a][b]);
                 }

@@ -173,13 +173,38 @@ __device__ __forceinline__ void row_sums(const double (&part)[R], double *red, d
 }
 
 // ---------------------------------------------------------------- sweep inverse
+// Copies the strictly lower register blocks onto the strictly upper ones:
+// element (I, J) of a block with b > a takes element (J, I), staged through
+// T (NMAX x (P+1) doubles).  Diagonal blocks are left as they are.
+template <int P, int R>
+__device__ __forceinline__ void mirror_lower(double (&A)[R][R], double *T, const Thr &t)
+{
+    constexpr int LD = P + 1;
+    static_for<0, R - 1>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if (r > 0) __syncthreads();   // T of the previous block column is read
+#pragma unroll
+        for (int a = r + 1; a < R; ++a) T[(t.tr + P * a) * LD + t.tc] = A[a][r];
+        __syncthreads();
+#pragma unroll
+        for (int b = r + 1; b < R; ++b) A[r][b] = T[(t.tc + P * b) * LD + t.tr];
+    });
+    __syncthreads();
+}
+
 // In-place inverse of an SPD matrix by the symmetric sweep operator
 // (Beaton / Goodnight): pivot k maps a_ij -> a_ij - a_ik (a_kj / a_kk),
 // a_ik -> a_ik / a_kk, a_kk -> -1 / a_kk; after every pivot the matrix holds
-// -inv(A), which is negated at the end.  Returns false (uniformly) if a pivot is not
-// positive, i.e. A is not positive definite -- the same test as dpotrf.
+// -inv(A), which is negated at the end.  The operator keeps the matrix
+// symmetric, so only the register blocks with b <= a are swept: column k is
+// assembled from the owners of its lower part (rows in blocks >= k's block)
+// and of row k (columns in blocks < k's block), and the strictly upper blocks
+// are mirrored from the lower ones at the end (T: NMAX x (P+1) doubles).
+// Returns false (uniformly) if a pivot is not positive, i.e. A is not positive
+// definite -- the same test as dpotrf.
 template <int P, int R>
-__device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *colbuf, const Thr &t)
+__device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *colbuf, double *T,
+                                              const Thr &t)
 {
     constexpr int NMAX = P * R;
     int buf = 0;
@@ -193,7 +218,11 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
             double *cb = colbuf + buf * NMAX;
             if (t.tc == kt) {
 #pragma unroll
-                for (int a = 0; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
+                for (int a = kb; a < R; ++a) cb[t.tr + P * a] = A[a][kb];
+            }
+            if (t.tr == kt) {
+#pragma unroll
+                for (int b = 0; b < kb; ++b) cb[t.tc + P * b] = A[kb][b];
             }
             __syncthreads();
             const double d = cb[k];
@@ -213,14 +242,14 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
 #pragma unroll
             for (int a = 0; a < R; ++a)
 #pragma unroll
-                for (int b = 0; b < R; ++b) A[a][b] = fma(-cI[a], tJ[b], A[a][b]);
+                for (int b = 0; b <= a; ++b) A[a][b] = fma(-cI[a], tJ[b], A[a][b]);
             if (t.tc == kt) {
 #pragma unroll
-                for (int a = 0; a < R; ++a) A[a][kb] = cI[a] * dinv;
+                for (int a = kb; a < R; ++a) A[a][kb] = cI[a] * dinv;
             }
             if (t.tr == kt) {
 #pragma unroll
-                for (int b = 0; b < R; ++b) A[kb][b] = tJ[b];
+                for (int b = 0; b <= kb; ++b) A[kb][b] = tJ[b];
             }
             if (t.tr == kt && t.tc == kt) A[kb][kb] = -dinv;
             if (!good) { ok = false; break; }
@@ -231,7 +260,8 @@ __device__ __forceinline__ bool sweep_inverse(double (&A)[R][R], int n, double *
 #pragma unroll
     for (int a = 0; a < R; ++a)
 #pragma unroll
-        for (int b = 0; b < R; ++b) A[a][b] = -A[a][b];
+        for (int b = 0; b <= a; ++b) A[a][b] = -A[a][b];
+    mirror_lower<P, R>(A, T, t);
     return true;
 }
 

@@ -219,7 +219,7 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
                 const int I = t.tr + P * a, J = t.tc + P * b;
                 A[a][b] = (I == J) ? s_zd[I] : -A[a][b];   // Z = Diag(zd) - offdiag(L)
             }
-        if (!sweep_inverse<P, R>(A, n, s_col, t)) { status = ERR_CHOL_Z; break; }
+        if (!sweep_inverse<P, R>(A, n, s_col, s_scr, t)) { status = ERR_CHOL_Z; break; }
 #pragma unroll
         for (int a = 0; a < R; ++a)
             if (t.tr == t.tc) s_dg[t.tr + P * a] = A[a][a];
@@ -230,7 +230,7 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         for (int b = 0; b < R; ++b)
 #pragma unroll
             for (int a = 0; a < R; ++a) A[a][b] = A[a][b] * sld<P>(sX, a, b);
-        if (!sweep_inverse<P, R>(A, n, s_col, t)) { status = ERR_SCHUR; break; }
+        if (!sweep_inverse<P, R>(A, n, s_col, s_scr, t)) { status = ERR_SCHUR; break; }
         {
             double part[R];
 #pragma unroll

@@ -21,7 +21,7 @@ __global__ __launch_bounds__(P * P) void dist_selftest_kernel(int op, int n, int
     double A[R][R];
     dload<P, R>(A, si);
     bool ok = true;
-    if (op == 0) ok = sweep_inverse<P, R>(A, n, s_col, t);
+    if (op == 0) ok = sweep_inverse<P, R>(A, n, s_col, s_T, t);
     else if (op == 1) ok = chol_lower<P, R>(A, n, s_col, t);
     else if (op == 2) ok = chol_test<P, R>(A, n, s_col, t);
     else if (op == 3) symmetrize<P, R>(A, s_T, t);

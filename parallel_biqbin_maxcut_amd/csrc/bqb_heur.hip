@@ -108,8 +108,13 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     const auto spl = [&](int j, int i) -> double {
         return (FL && use_f) ? (double)s_spl[j + i * n0] : p.SP_L[j + (size_t)i * n0];
     };
-    const auto lv = [&](int i, int j) -> double {
-        return (FL && use_f) ? (double)s_lf[i + j * n] : L[i + (size_t)j * ld];
+    const auto splx = [&](auto uf, int j, int i) -> double {
+        if constexpr (decltype(uf)::value) return (double)s_spl[j + i * n0];
+        else return p.SP_L[j + (size_t)i * n0];
+    };
+    const auto lvx = [&](auto uf, int i, int j) -> double {
+        if constexpr (decltype(uf)::value) return (double)s_lf[i + j * n];
+        else return L[i + (size_t)j * ld];
     };
     // first random cut xh in {-1,1}^n (heuristic.c:38-39)
     for (int i = t.tid; i < NMAX; i += NT)
@@ -171,94 +176,100 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                 s_sign[hh][i] = (sca < 0) ? -1 : 1;
             }
             __syncthreads();
-            // 1-opt (heuristic.c:195-296), one wave per hyperplane: the cut
-            // stays in LDS, Lx in registers (row i = lane + 64 q)
-            for (int hh = wave; hh < hb; hh += NW) {
-                signed char *xcur = s_sign[hh];
-                // Lx_i = sum_j L(i, j) x_j in j order; L is symmetric, so
-                // column j is read (lanes on consecutive rows)
-                double Lx[PL];
+            // 1-opt and cut values with the L / SP_L storage fixed at compile
+            // time, so the loads of the loops are scheduled together
+            const auto opt_eval = [&](auto uf) {
+                // 1-opt (heuristic.c:195-296), one wave per hyperplane: the cut
+                // stays in LDS, Lx in registers (row i = lane + 64 q)
+                for (int hh = wave; hh < hb; hh += NW) {
+                    signed char *xcur = s_sign[hh];
+                    // Lx_i = sum_j L(i, j) x_j in j order; L is symmetric, so
+                    // column j is read (lanes on consecutive rows)
+                    double Lx[PL];
 #pragma unroll
-                for (int q = 0; q < PL; ++q) Lx[q] = 0.0;
-                for (int j = 0; j < n; ++j) {
-                    const double xj = (double)xcur[j];
+                    for (int q = 0; q < PL; ++q) Lx[q] = 0.0;
+                    for (int j = 0; j < n; ++j) {
+                        const double xj = (double)xcur[j];
 #pragma unroll
-                    for (int q = 0; q < PL; ++q) {
-                        const int i = lane + 64 * q;
-                        if (i < n) Lx[q] += lv(i, j) * xj;
-                    }
-                }
-                double bv = -1e9;
-                int bi = 0x7fffffff;
-#pragma unroll
-                for (int q = 0; q < PL; ++q) {
-                    const int i = lane + 64 * q;
-                    if (i < n) {
-                        const double dl = lv(i, i) - xcur[i] * Lx[q];
-                        if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
-                    }
-                }
-                wave_argmax(bv, bi);
-                int flips = 0;
-                while (bv > 0.001 && flips < 100 * n + 100) {
-                    ++flips;
-                    const int index = bi;
-                    const int xi = xcur[index];
-#pragma unroll
-                    for (int q = 0; q < PL; ++q) {
-                        const int i = lane + 64 * q;
-                        if (i < n) {
-                            const double lij = lv(i, index);   // = L(index, i)
-                            if (fabs(lij) > 0.001) Lx[q] = (xi > 0) ? Lx[q] - 2 * lij : Lx[q] + 2 * lij;
+                        for (int q = 0; q < PL; ++q) {
+                            const int i = lane + 64 * q;
+                            if (i < n) Lx[q] += lvx(uf, i, j) * xj;
                         }
                     }
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == (index & 63)) xcur[index] = (signed char)(-xi);
-                    __builtin_amdgcn_wave_barrier();
-                    bv = -1e9;
-                    bi = 0x7fffffff;
+                    double bv = -1e9;
+                    int bi = 0x7fffffff;
 #pragma unroll
                     for (int q = 0; q < PL; ++q) {
                         const int i = lane + 64 * q;
                         if (i < n) {
-                            const double dl = lv(i, i) - xcur[i] * Lx[q];
+                            const double dl = lvx(uf, i, i) - xcur[i] * Lx[q];
                             if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
                         }
                     }
                     wave_argmax(bv, bi);
-                }
-                // lift to the original vertices (fixed ones keep their value)
-                for (int i = lane; i < B; i += 64) s_lift[hh][i] = xf[i] ? (signed char)sx[i] : (signed char)0;
-                __builtin_amdgcn_wave_barrier();
-                for (int a = lane; a < n - 1; a += 64) s_lift[hh][s_free[a]] = (signed char)((xcur[a] + 1) / 2);
-                __builtin_amdgcn_wave_barrier();
-                // its cut value (update_best -> evaluateSolution): the sum of
-                // SP_L over the pairs with sol_i = sol_j = 1, lanes over j and
-                // the 0/1 values as factors; integer data, so the sum is exact
-                // whatever the order
-                double acc = 0.0;
-                {
-                    double yj[(NMAX_GW + 63) / 64];
+                    int flips = 0;
+                    while (bv > 0.001 && flips < 100 * n + 100) {
+                        ++flips;
+                        const int index = bi;
+                        const int xi = xcur[index];
 #pragma unroll
-                    for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
-                        const int j = lane + 64 * q;
-                        yj[q] = (j < B) ? (double)s_lift[hh][j] : 0.0;
+                        for (int q = 0; q < PL; ++q) {
+                            const int i = lane + 64 * q;
+                            if (i < n) {
+                                const double lij = lvx(uf, i, index);   // = L(index, i)
+                                if (fabs(lij) > 0.001) Lx[q] = (xi > 0) ? Lx[q] - 2 * lij : Lx[q] + 2 * lij;
+                            }
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane == (index & 63)) xcur[index] = (signed char)(-xi);
+                        __builtin_amdgcn_wave_barrier();
+                        bv = -1e9;
+                        bi = 0x7fffffff;
+#pragma unroll
+                        for (int q = 0; q < PL; ++q) {
+                            const int i = lane + 64 * q;
+                            if (i < n) {
+                                const double dl = lvx(uf, i, i) - xcur[i] * Lx[q];
+                                if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
+                            }
+                        }
+                        wave_argmax(bv, bi);
                     }
-#pragma unroll 4
-                    for (int i = 0; i < B; ++i) {
-                        const double yi = (double)s_lift[hh][i];
-                        double r = 0.0;
+                    // lift to the original vertices (fixed ones keep their value)
+                    for (int i = lane; i < B; i += 64) s_lift[hh][i] = xf[i] ? (signed char)sx[i] : (signed char)0;
+                    __builtin_amdgcn_wave_barrier();
+                    for (int a = lane; a < n - 1; a += 64) s_lift[hh][s_free[a]] = (signed char)((xcur[a] + 1) / 2);
+                    __builtin_amdgcn_wave_barrier();
+                    // its cut value (update_best -> evaluateSolution): the sum of
+                    // SP_L over the pairs with sol_i = sol_j = 1, lanes over j and
+                    // the 0/1 values as factors; integer data, so the sum is exact
+                    // whatever the order
+                    double acc = 0.0;
+                    {
+                        double yj[(NMAX_GW + 63) / 64];
 #pragma unroll
                         for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
                             const int j = lane + 64 * q;
-                            if (64 * q < B) r += (j < B) ? yj[q] * spl(j, i) : 0.0;
+                            yj[q] = (j < B) ? (double)s_lift[hh][j] : 0.0;
                         }
-                        acc += yi * r;
+#pragma unroll 4
+                        for (int i = 0; i < B; ++i) {
+                            const double yi = (double)s_lift[hh][i];
+                            double r = 0.0;
+#pragma unroll
+                            for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
+                                const int j = lane + 64 * q;
+                                if (64 * q < B) r += (j < B) ? yj[q] * splx(uf, j, i) : 0.0;
+                            }
+                            acc += yi * r;
+                        }
                     }
+                    acc = wave_sum(acc);
+                    if (lane == 0) s_val[hh] = acc;
                 }
-                acc = wave_sum(acc);
-                if (lane == 0) s_val[hh] = acc;
-            }
+            };
+            if (FL && use_f) opt_eval(std::true_type{});
+            else opt_eval(std::false_type{});
             __syncthreads();
             // update_best: first strictly better hyperplane in order
             if (t.tid == 0) {

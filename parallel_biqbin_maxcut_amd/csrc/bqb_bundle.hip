@@ -476,13 +476,16 @@ __global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs
 
 // Column / slab copy inside one node's bundle (memmove order preserved by
 // the caller: destinations precede sources).
-__device__ void bundle_move(const BatchDims &d, const BatchPtrs &p, int node, int m, int from, int to)
+__device__ void bundle_move(const BatchDims &d, const BatchPtrs &p, int node, int m, int n, int from, int to)
 {
     const size_t slab = (size_t)d.ld * d.ld;
     double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
     double *Xb = p.Xb + (size_t)node * d.max_bundle * slab;
     for (int l = threadIdx.x; l < m; l += BT) G[(size_t)to * m + l] = G[(size_t)from * m + l];
-    for (size_t e = threadIdx.x; e < slab; e += BT) Xb[to * slab + e] = Xb[from * slab + e];
+    for (InteriorWalk w(n, threadIdx.x, BT); w.more(); w.next()) {
+        const size_t e = w.off(d.ld);
+        Xb[to * slab + e] = Xb[from * slab + e];
+    }
     if (threadIdx.x == 0) p.F[(size_t)node * d.max_bundle + to] = p.F[(size_t)node * d.max_bundle + from];
     __syncthreads();
 }
@@ -546,7 +549,8 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
         // X = X_bundle * lambda, each entry summed over the bundle in order
         double *X = p.X + node * slab;
         const double *Xb = p.Xb + (size_t)node * d.max_bundle * slab;
-        for (size_t e = threadIdx.x; e < slab; e += BT) {
+        for (InteriorWalk w(s.n, threadIdx.x, BT); w.more(); w.next()) {
+            const size_t e = w.off(d.ld);
             double acc = 0.0;
             for (int j = 0; j < k; ++j) acc += Xb[j * slab + e] * s_lam[j];
             X[e] = acc;
@@ -562,7 +566,7 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
     for (int i = 0; i < kp; ++i) {
         if (s_lam[i] < 0.01 * lmax) ++subtracted;
         else {
-            if (i > next) bundle_move(d, p, node, m, i, next);
+            if (i > next) bundle_move(d, p, node, m, s.n, i, next);
             ++next;
         }
     }
@@ -574,12 +578,15 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
     int pos;
     if (serious) pos = bundle;
     else {
-        bundle_move(d, p, node, m, k - 1, bundle);
+        bundle_move(d, p, node, m, s.n, k - 1, bundle);
         pos = bundle - 1;
     }
     for (int l = threadIdx.x; l < m; l += BT) G[(size_t)pos * m + l] = g[l];
     double *Xdst = p.Xb + ((size_t)node * d.max_bundle + pos) * slab;
-    for (size_t e = threadIdx.x; e < slab; e += BT) Xdst[e] = Xt[e];
+    for (InteriorWalk w(s.n, threadIdx.x, BT); w.more(); w.next()) {
+        const size_t e = w.off(d.ld);
+        Xdst[e] = Xt[e];
+    }
     // F = <L, X_test> (bundle.c:152-163)
     double acc = 0.0;
     const double *L = p.L + node * slab;

@@ -270,7 +270,10 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         __syncthreads();
         for (int c = threadIdx.x; c < m; c += CT) G[c] = g[c];
         double *Xb = p.Xb + (size_t)node * d.max_bundle * slab;
-        for (int e = threadIdx.x; e < (int)slab; e += CT) Xb[e] = X[e];
+        for (InteriorWalk w(s.n, threadIdx.x, CT); w.more(); w.next()) {
+            const size_t e = w.off(d.ld);
+            Xb[e] = X[e];
+        }
         const double F0 = dot_LX(p.L + node * slab, X, s.n, d.ld, red);
         if (threadIdx.x == 0) {
             p.F[(size_t)node * d.max_bundle] = F0;
@@ -418,7 +421,10 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         __syncthreads();
         for (int c = threadIdx.x; c < m; c += CT) G[(size_t)s.bundle * m + c] = g[c];
         double *Xdst = p.Xb + ((size_t)node * d.max_bundle + s.bundle) * slab;
-        for (int e = threadIdx.x; e < (int)slab; e += CT) Xdst[e] = Xt[e];
+        for (InteriorWalk w(s.n, threadIdx.x, CT); w.more(); w.next()) {
+            const size_t e = w.off(d.ld);
+            Xdst[e] = Xt[e];
+        }
         const double Fb = dot_LX(p.L + node * slab, Xt, s.n, d.ld, red);
         if (threadIdx.x == 0) {
             p.F[(size_t)node * d.max_bundle + s.bundle] = Fb;

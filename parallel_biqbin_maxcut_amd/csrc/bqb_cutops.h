@@ -62,4 +62,28 @@ __device__ __forceinline__ int pair_id(int r, int c)
     return c * (c - 1) / 2 + r;
 }
 
+// Walks the n x n interior of a column-major slab (leading dimension ld)
+// with nt threads: thread t visits flat indices t, t + nt, ... of the n x n
+// block (column by column) and off() gives the slab offset r + c ld.  The
+// padding outside the interior is never read or written by these walks.
+struct InteriorWalk {
+    int r, c, dr, dc, n;
+    __device__ InteriorWalk(int n_, int t, int nt) : n(n_)
+    {
+        if (n_ <= 0) { r = dr = dc = 0; c = n_; return; }
+        c = t / n_;
+        r = t - c * n_;
+        dc = nt / n_;
+        dr = nt - dc * n_;
+    }
+    __device__ bool more() const { return c < n; }
+    __device__ size_t off(int ld) const { return (size_t)r + (size_t)c * ld; }
+    __device__ void next()
+    {
+        r += dr;
+        c += dc;
+        if (r >= n) { r -= n; ++c; }
+    }
+};
+
 }  // namespace bqb

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     __shared__ double s_best;
     __shared__ int s_flag;
     __shared__ int s_exact;
+    __shared__ int s_quarter;
 
     const int node = blockIdx.x;
     if (node >= d.nodes || !p.mask[node]) return;
@@ -83,15 +84,21 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         for (int i = 0; i < B; ++i)
             if (!xf[i]) s_free[nf++] = i;
         s_exact = FL ? 1 : 0;
+        s_quarter = FL ? 1 : 0;
     }
     float *const s_spl = s_f, *const s_lf = s_f + (FL ? n0 * n0 : 0);
     if (FL) {
         __syncthreads();
-        int bad = 0;
+        // bad: an entry that float does not hold exactly; frac: one that is
+        // not a multiple of 1/4 of modest size (then sums over the entries
+        // are not all exact)
+        int bad = 0, frac = 0;
+        const auto quarter = [](double v) { return 4.0 * v == rint(4.0 * v) && fabs(v) <= 1048576.0; };
         for (int e = t.tid; e < n0 * n0; e += NT) {
             const double v = p.SP_L[e];
             const float f = (float)v;
             bad |= ((double)f != v);
+            frac |= !quarter(v);
             s_spl[e] = f;
         }
         for (int e = t.tid; e < n * n; e += NT) {
@@ -99,9 +106,11 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
             const double v = L[i + (size_t)j * ld];
             const float f = (float)v;
             bad |= ((double)f != v);
+            frac |= !quarter(v);
             s_lf[e] = f;
         }
         if (bad) s_exact = 0;
+        if (frac) s_quarter = 0;
     }
     // use_f is read from s_exact after the barrier below
     bool use_f = false;
@@ -116,11 +125,15 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         if constexpr (decltype(uf)::value) return (double)s_lf[i + j * n];
         else return L[i + (size_t)j * ld];
     };
+    const auto lv0 = [&](int i, int j) -> double { return lvx(std::true_type{}, i, j); };
     // first random cut xh in {-1,1}^n (heuristic.c:38-39)
     for (int i = t.tid; i < NMAX; i += NT)
         s_xh[i] = (i < n) ? (double)(2 * (int)(bqb_draw(key, (uint64_t)i) >> 63) - 1) : 0.0;
     __syncthreads();
     use_f = FL && s_exact;
+    // quarter-integer data: every sum below is exact, so a cut's value is
+    // the quadratic form of the subproblem plus a per-node constant (below)
+    const bool use_q = use_f && s_quarter;
 
     // fh = value of its lift (heuristic.c:42-54)
     if (t.tid == 0) {
@@ -140,6 +153,28 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                 if (s_sol[j]) acc += spl(j, i);
         }
         fh = wg_sum<P>(acc, s_red, t);
+    }
+    // The lift of x (free vertex a takes (x_a + 1) / 2) has the value
+    // y'Ly + cq, where y is x with y_{n-1} = +1 (createSubproblem folds the
+    // fixed vertices into the last row and column of L and the constant
+    // cq).  With exact sums cq is read off the first cut: cq = fh - y'Ly.
+    // Flipping coordinate n-1 of x changes the form by -4 x_{n-1} (Lx)_{n-1}
+    // + 4 L(n-1, n-1).
+    double cq = 0.0;
+    if (use_q) {
+        double acc = 0.0, accl = 0.0;
+        for (int i = wave; i < n; i += NW) {
+            double r = 0.0;
+            for (int j = lane; j < n; j += 64) r += lv0(j, i) * s_xh[j];
+            r = wave_sum(r);
+            acc += s_xh[i] * r;
+            if (i == n - 1) accl = r;
+        }
+        if (lane != 0) acc = accl = 0.0;
+        double q = wg_sum<P>(acc, s_red, t);
+        const double lxl = wg_sum<P>(accl, s_red, t);
+        if (s_xh[n - 1] < 0) q = q + 4.0 * lxl + 4.0 * lv0(n - 1, n - 1);
+        cq = fh - q;
     }
 
     double heur_val = -1e9;
@@ -235,36 +270,51 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                         }
                         wave_argmax(bv, bi);
                     }
-                    // lift to the original vertices (fixed ones keep their value)
-                    for (int i = lane; i < B; i += 64) s_lift[hh][i] = xf[i] ? (signed char)sx[i] : (signed char)0;
-                    __builtin_amdgcn_wave_barrier();
-                    for (int a = lane; a < n - 1; a += 64) s_lift[hh][s_free[a]] = (signed char)((xcur[a] + 1) / 2);
-                    __builtin_amdgcn_wave_barrier();
-                    // its cut value (update_best -> evaluateSolution): the sum of
-                    // SP_L over the pairs with sol_i = sol_j = 1, lanes over j and
-                    // the 0/1 values as factors; integer data, so the sum is exact
-                    // whatever the order
                     double acc = 0.0;
-                    {
-                        double yj[(NMAX_GW + 63) / 64];
+                    if (use_q) {
+                        // the quadratic form from the 1-opt's Lx (exact sums)
+                        double tl = 0.0;
 #pragma unroll
-                        for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
-                            const int j = lane + 64 * q;
-                            yj[q] = (j < B) ? (double)s_lift[hh][j] : 0.0;
+                        for (int q = 0; q < PL; ++q) {
+                            const int i = lane + 64 * q;
+                            if (i < n) acc += (double)xcur[i] * Lx[q];
+                            if (i == n - 1) tl = Lx[q];
                         }
-#pragma unroll 4
-                        for (int i = 0; i < B; ++i) {
-                            const double yi = (double)s_lift[hh][i];
-                            double r = 0.0;
+                        acc = wave_sum(acc);
+                        tl = wave_sum(tl);
+                        if (xcur[n - 1] < 0) acc = acc + 4.0 * tl + 4.0 * lvx(uf, n - 1, n - 1);
+                        acc = acc + cq;
+                    } else {
+                        // lift to the original vertices (fixed ones keep their value)
+                        for (int i = lane; i < B; i += 64) s_lift[hh][i] = xf[i] ? (signed char)sx[i] : (signed char)0;
+                        __builtin_amdgcn_wave_barrier();
+                        for (int a = lane; a < n - 1; a += 64) s_lift[hh][s_free[a]] = (signed char)((xcur[a] + 1) / 2);
+                        __builtin_amdgcn_wave_barrier();
+                        // its cut value (update_best -> evaluateSolution): the sum of
+                        // SP_L over the pairs with sol_i = sol_j = 1, lanes over j and
+                        // the 0/1 values as factors; integer data, so the sum is exact
+                        // whatever the order
+                        {
+                            double yj[(NMAX_GW + 63) / 64];
 #pragma unroll
                             for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
                                 const int j = lane + 64 * q;
-                                if (64 * q < B) r += (j < B) ? yj[q] * splx(uf, j, i) : 0.0;
+                                yj[q] = (j < B) ? (double)s_lift[hh][j] : 0.0;
                             }
-                            acc += yi * r;
+#pragma unroll 4
+                            for (int i = 0; i < B; ++i) {
+                                const double yi = (double)s_lift[hh][i];
+                                double r = 0.0;
+#pragma unroll
+                                for (int q = 0; q < (NMAX_GW + 63) / 64; ++q) {
+                                    const int j = lane + 64 * q;
+                                    if (64 * q < B) r += (j < B) ? yj[q] * splx(uf, j, i) : 0.0;
+                                }
+                                acc += yi * r;
+                            }
                         }
+                        acc = wave_sum(acc);
                     }
-                    acc = wave_sum(acc);
                     if (lane == 0) s_val[hh] = acc;
                 }
             };

@@ -64,9 +64,18 @@ def test_setup_and_basic_ipm_match_oracle():
                 idx += 1
 
 
-def test_heuristic_matches_oracle_on_device_X():
-    SP = instance(60, 0.5, "unit", 1)
-    xf, sx = I.random_fixings(60, 8, 3, 2)
+@pytest.mark.parametrize("n0,weights,nfix,scale", [
+    (60, "unit", 8, 1.0),
+    (100, "pm1", 30, 1.0),
+    (80, "int10", 60, 1.0),
+    (70, "zpm1", 69, 1.0),
+    # entries that are not multiples of 1/4: cut values by full evaluation
+    (60, "int10", 20, 0.3),
+    (60, "int10", 20, 0.125),
+])
+def test_heuristic_matches_oracle_on_device_X(n0, weights, nfix, scale):
+    SP = instance(n0, 0.5, weights, 1) * scale
+    xf, sx = I.random_fixings(n0, nfix, 3, 2)
     ctx = api.BoundContext(SP, max_nodes=3)
     seeds = np.array([11, 12, 13], dtype=np.uint64)
     basic(ctx, xf, sx, LB=-1e9, seeds=seeds)
@@ -78,11 +87,17 @@ def test_heuristic_matches_oracle_on_device_X():
         X = ctx.matrix("X", b, n)
         val, x, status = O.heuristic_on(SP, xf[b], sx[b], PL, X, O.key(int(seeds[b]), 1000))
         assert status == 0 and st.status == 0
-        xd = ctx.get("xheur", b, np.int32, 59)
+        xd = ctx.get("xheur", b, np.int32, n0 - 1)
         assert np.array_equal(xd, x)
-        assert st.LB == val and st.improved == 1
-        assert np.array_equal(ctx.get("xbest", b, np.int32, 59), x)
-        assert O.evaluate_solution(SP, xd) == val
+        assert np.array_equal(ctx.get("xbest", b, np.int32, n0 - 1), x)
+        assert st.improved == 1
+        if scale == 1.0:
+            # integer data: every cut value is an exact sum
+            assert st.LB == val
+            assert O.evaluate_solution(SP, xd) == val
+        else:
+            # real weights: values summed in another order, same cut
+            assert abs(st.LB - val) <= 1e-12 * abs(val)
 
 
 @pytest.mark.parametrize("n0,thr", [(40, 0.05), (100, 0.05), (100, 1e-3)])

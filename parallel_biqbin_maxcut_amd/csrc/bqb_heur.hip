@@ -202,13 +202,28 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
             }
             __syncthreads();
             // sign(Z v) per (hyperplane, row), products summed in j order
-            // without fused multiply-add (as the reference's scalar loop)
-            for (int e = t.tid; e < hb * n; e += NT) {
-                const int hh = e / n, i = e - hh * n;
-                double sca = 0.0;
-#pragma unroll 8
-                for (int j = 0; j <= i; ++j) sca = __dadd_rn(sca, __dmul_rn(s_v[hh][j], Zs[i + (size_t)j * ld]));
-                s_sign[hh][i] = (sca < 0) ? -1 : 1;
+            // without fused multiply-add (as the reference's scalar loop); a
+            // thread takes one row for HG hyperplanes, so each entry of Z
+            // is loaded once for all of them (rows past hb are discarded)
+            {
+                constexpr int HG = 4;
+                static_assert(GW_BATCH % HG == 0, "hyperplane groups");
+                const int ng = (hb + HG - 1) / HG;
+                for (int e = t.tid; e < ng * n; e += NT) {
+                    const int g = e / n, i = e - g * n;
+                    double sca[HG];
+#pragma unroll
+                    for (int h = 0; h < HG; ++h) sca[h] = 0.0;
+#pragma unroll 4
+                    for (int j = 0; j <= i; ++j) {
+                        const double z = Zs[i + (size_t)j * ld];
+#pragma unroll
+                        for (int h = 0; h < HG; ++h) sca[h] = __dadd_rn(sca[h], __dmul_rn(s_v[g * HG + h][j], z));
+                    }
+#pragma unroll
+                    for (int h = 0; h < HG; ++h)
+                        if (g * HG + h < hb) s_sign[g * HG + h][i] = (sca[h] < 0) ? -1 : 1;
+                }
             }
             __syncthreads();
             // 1-opt and cut values with the L / SP_L storage fixed at compile
@@ -231,13 +246,21 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                             if (i < n) Lx[q] += lvx(uf, i, j) * xj;
                         }
                     }
+                    // the diagonal and the cut stay in registers for the flips
+                    double dg[PL], xr[PL];
+#pragma unroll
+                    for (int q = 0; q < PL; ++q) {
+                        const int i = lane + 64 * q;
+                        dg[q] = (i < n) ? lvx(uf, i, i) : 0.0;
+                        xr[q] = (i < n) ? (double)xcur[i] : 0.0;
+                    }
                     double bv = -1e9;
                     int bi = 0x7fffffff;
 #pragma unroll
                     for (int q = 0; q < PL; ++q) {
                         const int i = lane + 64 * q;
                         if (i < n) {
-                            const double dl = lvx(uf, i, i) - xcur[i] * Lx[q];
+                            const double dl = dg[q] - xr[q] * Lx[q];
                             if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
                         }
                     }
@@ -258,13 +281,16 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                         __builtin_amdgcn_wave_barrier();
                         if (lane == (index & 63)) xcur[index] = (signed char)(-xi);
                         __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                        for (int q = 0; q < PL; ++q)
+                            if (lane + 64 * q == index) xr[q] = -xr[q];
                         bv = -1e9;
                         bi = 0x7fffffff;
 #pragma unroll
                         for (int q = 0; q < PL; ++q) {
                             const int i = lane + 64 * q;
                             if (i < n) {
-                                const double dl = lvx(uf, i, i) - xcur[i] * Lx[q];
+                                const double dl = dg[q] - xr[q] * Lx[q];
                                 if (dl > bv || (dl == bv && i < bi)) { bv = dl; bi = i; }
                             }
                         }

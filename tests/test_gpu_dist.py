@@ -48,6 +48,23 @@ def test_chol_test_detects_indefinite(tile, nmax):
 
 
 @pytest.mark.parametrize("tile,nmax", TILES)
+@pytest.mark.parametrize("n_off", [1, 2, 15, 16, 17, 31, 33, 48, 0])
+def test_chol_test_first_failing_pivot(tile, nmax, n_off):
+    """A matrix whose leading k x k blocks are PD up to pivot p and fail at
+    p: the test must fail, whichever block (and position in it) p falls in;
+    the same matrix cut to its first p rows must pass."""
+    n = nmax - 1 if n_off == 0 else min(nmax, n_off + 20)
+    p = n - 1 if n_off == 0 else min(n_off, n - 1)
+    rng = np.random.default_rng(100 + n_off)
+    L = np.tril(rng.standard_normal((n, n)) * 0.3) + np.diag(1.0 + rng.random(n))
+    D = np.ones(n)
+    D[p] = -0.5                      # pivot p of A = L D L' is negative
+    A = (L * D) @ L.T
+    assert api.dist_selftest(tile, 2, A)[1] == 0
+    assert api.dist_selftest(tile, 2, A[:p, :p])[1] == 1 if p > 0 else True
+
+
+@pytest.mark.parametrize("tile,nmax", TILES)
 def test_symmetrize(tile, nmax):
     n = nmax - 1
     A = np.random.default_rng(1).standard_normal((n, n))

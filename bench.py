@@ -8,7 +8,7 @@ full SDPbound (FP64 IPM + bundle over triangle / pentagonal / heptagonal cuts,
 GW heuristic).  Every node receives the incumbent and the root gap of the
 instance's root bound, as a B&B worker does.
 
-One step = 65536 such nodes streamed through six slot groups of 512 node
+One step = 65536 such nodes streamed through twelve slot groups of 512 node
 slots (bqb_bound_stream_multi: every launch covers one group's 512 slots, a
 finished node's slot is refilled after each outer round, as the B&B node
 queue runs it; each group has its own HIP stream and host thread, so one
@@ -35,9 +35,10 @@ import time
 
 # Each slot group drives its own HIP stream; HIP maps the streams of a
 # process onto GPU_MAX_HW_QUEUES hardware queues (4 by default), and streams
-# that share a queue serialize.  Six groups need at least seven queues.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# that share a queue serialize: twelve groups run on 24 queues (with as many
+# queues as groups some streams still share one and throughput drops).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 24:
+    os.environ["GPU_MAX_HW_QUEUES"] = "24"
 
 import numpy as np
 
@@ -138,7 +139,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nodes", type=int, default=512, help="node slots (subproblems per launch)")
     ap.add_argument("--per-step", type=int, default=65536, help="nodes bounded per step")
-    ap.add_argument("--groups", type=int, default=6, help="slot groups (streams) of --nodes slots each")
+    ap.add_argument("--groups", type=int, default=12, help="slot groups (streams) of --nodes slots each")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

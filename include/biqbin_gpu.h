@@ -216,7 +216,7 @@ int bqb_bound_stream(bqb_ctx *c, int count, const int *xfixed, const int *solX, 
  * Each launch still covers one context's slots.  Results are the same as
  * bqb_bound_stream's (a node's results depend only on its inputs).  The
  * contexts must not be used by other threads meanwhile. */
-#define BQB_MAX_GROUPS 8
+#define BQB_MAX_GROUPS 16
 int bqb_bound_stream_multi(bqb_ctx *const *ctx, int nctx, int count, const int *xfixed, const int *solX,
                            const double *LB, const double *diff, const int *is_root, const uint64_t *seeds,
                            double *bound, double *fracsol, int *xbest, double *LB_out, int *improved,

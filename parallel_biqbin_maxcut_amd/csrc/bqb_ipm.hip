@@ -203,6 +203,10 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
     }
 
     const int max_iters = args.max_iters > 0 ? args.max_iters : IPM_MAX_ITERS;
+    // set when the dual line search accepted the full step at its first
+    // trial: that trial inverted exactly the next iteration's Z (same
+    // diagonal zd + 1 * dy, same off-diagonal), so Zi is already in its slab
+    bool zi_ready = false;
     while (gap > 1e-2) {
         if (iter >= max_iters) {
             if (args.max_iters <= 0) status = ERR_IPM_ITERS;
@@ -211,19 +215,24 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         ++iter;
 
         // ---- Zi = inv(Z) --------------------------------------------------
-        dload<P, R>(A, sL);
+        if (zi_ready) {
+            dload<P, R>(A, sZi);
+        } else {
+            dload<P, R>(A, sL);
 #pragma unroll
-        for (int a = 0; a < R; ++a)
+            for (int a = 0; a < R; ++a)
 #pragma unroll
-            for (int b = 0; b < R; ++b) {
-                const int I = t.tr + P * a, J = t.tc + P * b;
-                A[a][b] = (I == J) ? s_zd[I] : -A[a][b];   // Z = Diag(zd) - offdiag(L)
-            }
-        if (!sweep_inverse<P, R>(A, n, s_col, s_scr, t)) { status = ERR_CHOL_Z; break; }
+                for (int b = 0; b < R; ++b) {
+                    const int I = t.tr + P * a, J = t.tc + P * b;
+                    A[a][b] = (I == J) ? s_zd[I] : -A[a][b];   // Z = Diag(zd) - offdiag(L)
+                }
+            if (!sweep_inverse<P, R>(A, n, s_col, s_scr, t)) { status = ERR_CHOL_Z; break; }
+            dstore<P, R>(A, sZi);
+        }
+        zi_ready = false;
 #pragma unroll
         for (int a = 0; a < R; ++a)
             if (t.tr == t.tc) s_dg[t.tr + P * a] = A[a][a];
-        dstore<P, R>(A, sZi);
 
         // ---- Mi = inv(Zi o X), dy1 = Mi (-e) ------------------------------
 #pragma unroll
@@ -328,6 +337,10 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
             if (alpha_p < 1.0) alpha_p *= 0.95;
         }
         // ---- dual line search: Z + a Diag(dy) psd --------------------------
+        // The first trial (a = 1) is tested by the sweep inverse: its pivots
+        // are the Cholesky test's (the same trailing updates), and when it
+        // passes, the step is taken in full and the inverse is the next
+        // iteration's Zi (not in the debug run, which reads the Zi slab).
         double alpha_d = 1.0;
         {
             int tr_ = 0;
@@ -342,7 +355,17 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
                         A[a][b] = (I == J) ? s_zd[I] + alpha_d * s_v[I] : -A[a][b];
                     }
                 ++tr_;
-                if (chol_test<P, R>(A, n, s_col, t)) { ok = true; break; }
+                if (tr_ == 1 && !args.dbg) {
+                    if (sweep_inverse<P, R>(A, n, s_col, s_scr, t)) {
+                        dstore<P, R>(A, sZi);
+                        zi_ready = true;
+                        ok = true;
+                        break;
+                    }
+                } else if (chol_test<P, R>(A, n, s_col, t)) {
+                    ok = true;
+                    break;
+                }
                 alpha_d *= 0.8;
             }
             trials += tr_;

@@ -42,7 +42,7 @@ struct bqb_ctx {
     BqbParams params;
     int max_nodes;
     int tile;           /* IPM register tile (P*100 + R) */
-    int heur_tile;      /* heuristic tile: 1024 threads where they fit */
+    int heur_tile;      /* heuristic tile (P*100 + R) */
     void *arena;
     size_t arena_bytes;
     hipStream_t stream;
@@ -140,7 +140,10 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     if (!tile) { bqb_set_error("n0 = %d exceeds the largest compiled tile", n0); return NULL; }
     /* the heuristic's 1-opt runs one hyperplane per wave: 16 waves (32 x 32
      * threads) where the 32 x R tile covers n0 */
-    const int heur_tile = (n0 <= 128) ? 3204 : (n0 <= 160) ? 3205 : (n0 <= 192) ? 3206 : (n0 <= 224) ? 3207 : 3208;
+    /* the heuristic takes 256-thread tiles up to n0 = 112 (so several other
+     * kernels share its CU), 1024 threads above */
+    const int heur_tile = (n0 <= 64) ? 1604 : (n0 <= 80) ? 1605 : (n0 <= 96) ? 1606 : (n0 <= 112) ? 1607
+                        : (n0 <= 128) ? 3204 : (n0 <= 160) ? 3205 : (n0 <= 192) ? 3206 : (n0 <= 224) ? 3207 : 3208;
     const int tile_ld = (tile / 100) * (tile % 100), heur_ld = (heur_tile / 100) * (heur_tile % 100);
     bqb_ctx *c = calloc(1, sizeof(bqb_ctx));
     if (!c) { bqb_set_error("calloc"); return NULL; }

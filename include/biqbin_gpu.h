@@ -263,8 +263,10 @@ void bqb_ctx_trajectory_clear(bqb_ctx *c);
  * slots 0..nodes-1, run one phase (0-14 control phases; 100 basic IPM,
  * 101 IPM on L0, 102 heuristic,
  * 103 triangle update, 104/105 pentagonal/heptagonal update, 106 op_Bt
- * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP,
- * 109 bundle step), run the whole bound, read results, and copy one node's
+ * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP
+ * (arg: an upper bound on every bundle size, 0 if unknown; a bound <= 16
+ * selects the small-LDS QP kernel), 109 bundle step), run the whole bound,
+ * read results, and copy one node's
  * device buffer (ids documented in bqb_sdpbound.c) to or from the host. */
 int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
                  const double *diff, const int *is_root, const uint64_t *seeds);

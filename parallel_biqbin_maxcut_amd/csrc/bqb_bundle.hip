@@ -43,7 +43,7 @@ __device__ __forceinline__ double bsum(double v, double *red)
 
 // Cholesky L L' = A of a k x k SPD matrix in LDS (column-major, ld k), one
 // wave, lane i owns row i.  Returns false if a pivot is not positive.
-__device__ bool wave_chol(double *A, int k, int lane)
+__device__ __forceinline__ bool wave_chol(double *A, int k, int lane)
 {
     for (int j = 0; j < k; ++j) {
         const double d = A[j + j * k];
@@ -63,7 +63,7 @@ __device__ bool wave_chol(double *A, int k, int lane)
 }
 
 // Solve (L L') x = b in place (b in LDS), one wave.
-__device__ void wave_cholsolve(const double *Lc, int k, double *b, int lane)
+__device__ __forceinline__ void wave_cholsolve(const double *Lc, int k, double *b, int lane)
 {
     for (int j = 0; j < k; ++j) {
         if (lane == j) b[j] = b[j] / Lc[j + j * k];
@@ -82,7 +82,7 @@ __device__ void wave_cholsolve(const double *Lc, int k, double *b, int lane)
 }
 
 // bundle.c:390-653 on one wave.  Q (k x k), c, lambda in LDS.
-__device__ int solve_lambda_wave(int k, const double *Q, const double *c, double *lam, double *A,
+__device__ __forceinline__ int solve_lambda_wave(int k, const double *Q, const double *c, double *lam, double *A,
                                  double *u, double *w, int lane)
 {
     const bool in = lane < k;
@@ -162,7 +162,7 @@ constexpr int KR = 32;
 constexpr int PE = BQB_QP_REG_PARTIALS;
 
 template <int K>
-__device__ int solve_lambda_reg(int k, const double *Q, const double *c, double *lam, double *T,
+__device__ __forceinline__ int solve_lambda_reg(int k, const double *Q, const double *c, double *lam, double *T,
                                 int lane)
 {
     const bool in = lane < k;
@@ -272,11 +272,20 @@ __device__ int solve_lambda_reg(int k, const double *Q, const double *c, double 
     return 0;
 }
 
-__global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
+// KM = the largest bundle the launch may hold: 64 (every case), or 16 when
+// the host knows that no node of the launch has more than 16 elements --
+// then Q and the LDS scratch shrink (20 KB instead of 68 KB) and four
+// workgroups fit on a CU.  Both variants do the same arithmetic in the same
+// order.
+template <int KM>
+__global__ __launch_bounds__(BT, (KM == KMAX) ? 2 : 3) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
 {
-    __shared__ double s_Q[KMAX * KMAX];
-    __shared__ double s_A[KMAX * KMAX];
-    __shared__ double s_zeta[KMAX], s_c[KMAX], s_lam[KMAX], s_u[KMAX], s_w[KMAX];
+    // s_A: the Gram pass's row tiles, the solver's factor, the wave partials
+    constexpr int SA = (KM == KMAX) ? KMAX * KMAX : 2048;
+    static_assert(SA >= KR * (KR + 1) || KM <= 16, "factor storage");
+    __shared__ double s_Q[KM * KM];
+    __shared__ double s_A[SA];
+    __shared__ double s_zeta[KM], s_c[KM], s_lam[KM], s_u[KM], s_w[KM];
     __shared__ double s_red[BT / 64];
     __shared__ int s_done, s_stat;
     __shared__ double s_t;
@@ -288,7 +297,7 @@ __global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs
     const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
     const int k = s.bundle;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (k > KMAX || k < 1) {
+    if (k > KM || k < 1) {
         if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
         return;
     }
@@ -317,11 +326,11 @@ __global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs
     // products and sums its rows in order l = 0..m-1.
     {
         const int kc = k + 1;                       // column k holds gamma
-        const int RS = min(256 + 1, (KMAX * KMAX) / kc);   // column stride (odd: bank spread)
+        const int RS = min(256 + 1, SA / kc);   // column stride (odd: bank spread)
         const int R = RS - 1;
         const int npq = k * (k + 1) / 2;
         const int np = npq + k;
-        constexpr int GP = (KMAX * (KMAX + 1) / 2 + KMAX + BT - 1) / BT;
+        constexpr int GP = (KM * (KM + 1) / 2 + KM + BT - 1) / BT;
         double acc[GP];
         int pa[GP], pb[GP];
         static_for<0, GP>([&](auto sc) {
@@ -381,10 +390,15 @@ __global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs
     while (!s_done) {
         ++cnt;
         if (wave == 0) {
-            const int st = (k <= 8)    ? solve_lambda_reg<8>(k, s_Q, s_c, s_lam, s_A, lane)
-                           : (k <= 16) ? solve_lambda_reg<16>(k, s_Q, s_c, s_lam, s_A, lane)
-                           : (k <= KR) ? solve_lambda_reg<KR>(k, s_Q, s_c, s_lam, s_A, lane)
-                                       : solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
+            int st;
+            if constexpr (KM <= 16)
+                st = (k <= 8) ? solve_lambda_reg<8>(k, s_Q, s_c, s_lam, s_A, lane)
+                              : solve_lambda_reg<16>(k, s_Q, s_c, s_lam, s_A, lane);
+            else
+                st = (k <= 8)    ? solve_lambda_reg<8>(k, s_Q, s_c, s_lam, s_A, lane)
+                   : (k <= 16) ? solve_lambda_reg<16>(k, s_Q, s_c, s_lam, s_A, lane)
+                   : (k <= KR) ? solve_lambda_reg<KR>(k, s_Q, s_c, s_lam, s_A, lane)
+                               : solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
             if (lane == 0 && st) s_stat = st;
         }
         __syncthreads();
@@ -450,16 +464,16 @@ __global__ __launch_bounds__(BT, 2) void bundle_qp_kernel(BatchDims d, BatchPtrs
             constexpr int j = decltype(jc)::value;
             if (j >= k) return;
             const double v = wsum(pe[j]);
-            if (lane == 0) s_A[wave * KMAX + j] = v;
+            if (lane == 0) s_A[wave * KM + j] = v;
         });
         for (int j = PE; j < k; ++j) {
             const double v = wsum(tmp[(size_t)(j - PE) * BT + threadIdx.x]);
-            if (lane == 0) s_A[wave * KMAX + j] = v;
+            if (lane == 0) s_A[wave * KM + j] = v;
         }
         __syncthreads();
         for (int j = threadIdx.x; j < k; j += BT) {
             double v = 0.0;
-            for (int w = 0; w < BT / 64; ++w) v += s_A[w * KMAX + j];
+            for (int w = 0; w < BT / 64; ++w) v += s_A[w * KM + j];
             s_c[j] = (m > 0) ? s_zeta[j] + (-t) * v : s_zeta[j];
         }
         for (int j = PE; j < k; ++j) tmp[(size_t)(j - PE) * BT + threadIdx.x] = 0.0;
@@ -616,9 +630,14 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
 
 }  // namespace bqb
 
-extern "C" hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, hipStream_t s)
+// kbound: an upper bound on the bundle size of every node of the launch
+// (0: unknown)
+extern "C" hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, int kbound, hipStream_t s)
 {
-    hipLaunchKernelGGL(bqb::bundle_qp_kernel, dim3(d->nodes), dim3(bqb::BT), 0, s, *d, *p);
+    if (kbound > 0 && kbound <= 16)
+        hipLaunchKernelGGL(bqb::bundle_qp_kernel<16>, dim3(d->nodes), dim3(bqb::BT), 0, s, *d, *p);
+    else
+        hipLaunchKernelGGL(bqb::bundle_qp_kernel<bqb::KMAX>, dim3(d->nodes), dim3(bqb::BT), 0, s, *d, *p);
     return hipGetLastError();
 }
 

@@ -33,7 +33,7 @@ hipError_t bqb_launch_opbt(const BatchDims *d, const BatchPtrs *p, int use_gtest
 hipError_t bqb_launch_trisep(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
 hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStream_t s);
 hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, hipStream_t s);
-hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
+hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, int kbound, hipStream_t s);
 hipError_t bqb_launch_bundle_step(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
 
 struct bqb_ctx {
@@ -344,7 +344,7 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     case 105: return bqb_check(bqb_launch_sa(d, p, 7, s), "sa7 launch");
     case 106: return bqb_check(bqb_launch_csr_build(d, p, s), "csr launch");
     case 107: return bqb_check(bqb_launch_opbt(d, p, arg, s), "opbt launch");
-    case 108: return bqb_check(bqb_launch_bundle_qp(d, p, s), "qp launch");
+    case 108: return bqb_check(bqb_launch_bundle_qp(d, p, arg, s), "qp launch");
     case 109: return bqb_check(bqb_launch_bundle_step(d, p, s), "step launch");
     default:
         if (what >= 0 && what <= PH_BUNDLE_NEXT)
@@ -573,10 +573,11 @@ int bqb_ctx_advance(bqb_ctx *c)
      * it is between rounds, runs up to init_bundle_iter of its bundle
      * iterations, and goes through the round end once its bundle iterations
      * are done (a round longer than the slice continues in the next call) */
-    int in_loop = 0, bmax = 0;
+    int in_loop = 0, bmax = 0, kmax = 0;
     const int slice = P->init_bundle_iter > 0 ? P->init_bundle_iter : 1;
     for (int i = 0; i < c->hi; ++i) {
         const NodeState *s = &c->h_st[i];
+        if (c->tag[i] >= 0 && s->bundle > kmax) kmax = s->bundle;
         if (c->tag[i] < 0 || !s->active || s->stage != BQB_STAGE_LOOP) continue;
         in_loop = 1;
         int left = s->bdl_left;
@@ -593,7 +594,10 @@ int bqb_ctx_advance(bqb_ctx *c)
         RUN(PH_OUTER_BEGIN, 0);
         for (int b = 0; b < nb; ++b) {
             RUN(PH_BUNDLE_NEXT, b);
-            RUN(108, 0);         /* QP -> gamma_test */
+            /* QP -> gamma_test; each bundle iteration appends at most one
+             * element, so no bundle exceeds kmax + b (the QP kernel sizes its
+             * LDS from this bound) */
+            RUN(108, kmax + b);
             RUN(107, 1);         /* L0 = L - B'(gamma_test) */
             RUN(101, 0);         /* IPM: L0 -> X_test */
             RUN(109, 0);         /* serious / null step */

@@ -30,7 +30,7 @@
 hipError_t bqb_launch_control(const BatchDims *d, const BatchPtrs *p, int phase, int arg, hipStream_t s);
 hipError_t bqb_launch_csr_build(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
 hipError_t bqb_launch_opbt(const BatchDims *d, const BatchPtrs *p, int use_gtest, hipStream_t s);
-hipError_t bqb_launch_trisep(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
+hipError_t bqb_launch_trisep(const BatchDims *d, const BatchPtrs *p, int tri_ineq, hipStream_t s);
 hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStream_t s);
 hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, hipStream_t s);
 hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, int kbound, hipStream_t s);
@@ -339,7 +339,7 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     case 100: return ipm_launch(c, p->L, p->X);
     case 101: return ipm_launch(c, p->L0, p->Xt);
     case 102: return bqb_check(bqb_launch_heur(d, p, c->heur_tile, s), "heur launch");
-    case 103: return bqb_check(bqb_launch_trisep(d, p, s), "trisep launch");
+    case 103: return bqb_check(bqb_launch_trisep(d, p, c->params.TriIneq, s), "trisep launch");
     case 104: return bqb_check(bqb_launch_sa(d, p, 5, s), "sa5 launch");
     case 105: return bqb_check(bqb_launch_sa(d, p, 7, s), "sa7 launch");
     case 106: return bqb_check(bqb_launch_csr_build(d, p, s), "csr launch");

@@ -36,10 +36,14 @@ __device__ __forceinline__ double tri_value(const double *X, int ld, int type, i
 
 __device__ __forceinline__ long long c3(long long n) { return n * (n - 1) * (n - 2) / 6; }
 
+// LM: the list capacity held in LDS -- TRI_LIST_MAX, or 1024 when TriIneq
+// is at most that (10 n0 for n0 <= 102), which halves the kernel's LDS so
+// that four workgroups fit on a CU.  The same arithmetic either way.
+template <int LM>
 __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
 {
-    __shared__ double s_lval[TRI_LIST_MAX];
-    __shared__ unsigned s_lidx[TRI_LIST_MAX];      // packed (type-1, ii, jj, kk)
+    __shared__ double s_lval[LM];
+    __shared__ unsigned s_lidx[LM];      // packed (type-1, ii, jj, kk)
     __shared__ double s_qval[CHUNK];
     __shared__ unsigned s_qidx[CHUNK];
     __shared__ int s_cnt[TT];
@@ -59,7 +63,7 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
     TriCut *cuts = p.tri + (size_t)node * d.max_tri;
     double *cy = p.tri_y + (size_t)node * d.max_tri;
     const double *gam = p.gamma + (size_t)node * d.max_m;
-    const int cap = P.TriIneq < TRI_LIST_MAX ? P.TriIneq : TRI_LIST_MAX;
+    const int cap = P.TriIneq < LM ? P.TriIneq : LM;
     const double thr = P.violated_TriIneq;
 
     // ---- 1. purge (cutting_planes.c:140-166), chunked stable compaction --
@@ -328,8 +332,12 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
 
 }  // namespace bqb
 
-extern "C" hipError_t bqb_launch_trisep(const BatchDims *d, const BatchPtrs *p, hipStream_t s)
+// tri_ineq: the TriIneq parameter of the launch's nodes (the list capacity)
+extern "C" hipError_t bqb_launch_trisep(const BatchDims *d, const BatchPtrs *p, int tri_ineq, hipStream_t s)
 {
-    hipLaunchKernelGGL(bqb::trisep_kernel, dim3(d->nodes), dim3(bqb::TT), 0, s, *d, *p);
+    if (tri_ineq <= 1024)
+        hipLaunchKernelGGL(bqb::trisep_kernel<1024>, dim3(d->nodes), dim3(bqb::TT), 0, s, *d, *p);
+    else
+        hipLaunchKernelGGL(bqb::trisep_kernel<bqb::TRI_LIST_MAX>, dim3(d->nodes), dim3(bqb::TT), 0, s, *d, *p);
     return hipGetLastError();
 }

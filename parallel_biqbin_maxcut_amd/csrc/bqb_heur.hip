@@ -33,14 +33,16 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// FL: the original objective SP_L and the node's L are copied into LDS as
-// float (dynamic shared memory) and read from there whenever every entry
-// converts exactly (integer weights give quarter-integers), so the 1-opt and
-// the cut evaluations read LDS instead of HBM/L2 with the same arithmetic.
-template <int P, int R, bool FL>
+// FL: LDS copies as float (dynamic shared memory), read whenever every
+// copied entry converts exactly (integer weights give quarter-integers), so
+// the 1-opt and the cut evaluations read LDS instead of HBM/L2 with the same
+// arithmetic.  FL = 1: the original objective SP_L and the node's L; FL = 2:
+// L only (for integer SP_L, whose cut values then come from the 1-opt's Lx
+// and which is read from memory for the two full evaluations per node).
+template <int P, int R, int FL>
 __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
 {
-    extern __shared__ float s_f[];     // FL: [n0*n0] SP_L, then [n*n] L
+    extern __shared__ float s_f[];     // FL 1: [n0*n0] SP_L, then [n*n] L; FL 2: [n*n] L
     constexpr int NT = P * P;
     constexpr int NW = NT / 64;
     constexpr int NMAX = P * R;
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         s_exact = FL ? 1 : 0;
         s_quarter = FL ? 1 : 0;
     }
-    float *const s_spl = s_f, *const s_lf = s_f + (FL ? n0 * n0 : 0);
+    float *const s_spl = s_f, *const s_lf = s_f + (FL == 1 ? n0 * n0 : 0);
     if (FL) {
         __syncthreads();
         // bad: an entry that float does not hold exactly; frac: one that is
@@ -97,9 +99,9 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
         for (int e = t.tid; e < n0 * n0; e += NT) {
             const double v = p.SP_L[e];
             const float f = (float)v;
-            bad |= ((double)f != v);
+            if (FL == 1) bad |= ((double)f != v);
             frac |= !quarter(v);
-            s_spl[e] = f;
+            if (FL == 1) s_spl[e] = f;
         }
         for (int e = t.tid; e < n * n; e += NT) {
             const int i = e % n, j = e / n;
@@ -115,10 +117,10 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     // use_f is read from s_exact after the barrier below
     bool use_f = false;
     const auto spl = [&](int j, int i) -> double {
-        return (FL && use_f) ? (double)s_spl[j + i * n0] : p.SP_L[j + (size_t)i * n0];
+        return (FL == 1 && use_f) ? (double)s_spl[j + i * n0] : p.SP_L[j + (size_t)i * n0];
     };
     const auto splx = [&](auto uf, int j, int i) -> double {
-        if constexpr (decltype(uf)::value) return (double)s_spl[j + i * n0];
+        if constexpr (decltype(uf)::value && FL == 1) return (double)s_spl[j + i * n0];
         else return p.SP_L[j + (size_t)i * n0];
     };
     const auto lvx = [&](auto uf, int i, int j) -> double {
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
     for (int i = t.tid; i < NMAX; i += NT)
         s_xh[i] = (i < n) ? (double)(2 * (int)(bqb_draw(key, (uint64_t)i) >> 63) - 1) : 0.0;
     __syncthreads();
-    use_f = FL && s_exact;
+    use_f = FL != 0 && s_exact;
     // quarter-integer data: every sum below is exact, so a cut's value is
     // the quadratic form of the subproblem plus a per-node constant (below)
     const bool use_q = use_f && s_quarter;
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
                     if (lane == 0) s_val[hh] = acc;
                 }
             };
-            if (FL && use_f) opt_eval(std::true_type{});
+            if (FL != 0 && use_f) opt_eval(std::true_type{});
             else opt_eval(std::false_type{});
             __syncthreads();
             // update_best: first strictly better hyperplane in order
@@ -420,37 +422,42 @@ __global__ __launch_bounds__(P * P) void heur_kernel(BatchDims d, BatchPtrs p)
 }
 
 template <int P, int R>
-static hipError_t launch_heur(const BatchDims &d, const BatchPtrs &p, hipStream_t s)
+static hipError_t launch_heur(const BatchDims &d, const BatchPtrs &p, int spl_integer, hipStream_t s)
 {
-    // float copies of SP_L and L (n <= n0) while they fit beside the static LDS
+    // float copies of L (and of SP_L unless it is integer-valued; n <= n0)
+    // while they fit beside the static LDS
     static const size_t static_lds = []() {
         hipFuncAttributes a;
-        return (hipFuncGetAttributes(&a, (const void *)heur_kernel<P, R, true>) == hipSuccess)
+        return (hipFuncGetAttributes(&a, (const void *)heur_kernel<P, R, 1>) == hipSuccess)
                    ? a.sharedSizeBytes : (size_t)160 * 1024;
     }();
-    const size_t dyn = sizeof(float) * 2 * (size_t)d.n0 * d.n0;
-    if (static_lds + dyn <= (size_t)160 * 1024)
-        hipLaunchKernelGGL((heur_kernel<P, R, true>), dim3(d.nodes), dim3(P * P), dyn, s, d, p);
+    const size_t one = sizeof(float) * (size_t)d.n0 * d.n0;
+    if (spl_integer && static_lds + one <= (size_t)160 * 1024)
+        hipLaunchKernelGGL((heur_kernel<P, R, 2>), dim3(d.nodes), dim3(P * P), one, s, d, p);
+    else if (static_lds + 2 * one <= (size_t)160 * 1024)
+        hipLaunchKernelGGL((heur_kernel<P, R, 1>), dim3(d.nodes), dim3(P * P), 2 * one, s, d, p);
     else
-        hipLaunchKernelGGL((heur_kernel<P, R, false>), dim3(d.nodes), dim3(P * P), 0, s, d, p);
+        hipLaunchKernelGGL((heur_kernel<P, R, 0>), dim3(d.nodes), dim3(P * P), 0, s, d, p);
     return hipGetLastError();
 }
 
 }  // namespace bqb
 
-extern "C" hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, hipStream_t s)
+// spl_integer: every entry of SP_L is an integer of magnitude <= 2^20
+extern "C" hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, int spl_integer,
+                                      hipStream_t s)
 {
     using namespace bqb;
     switch (tile) {
-    case 1604: return launch_heur<16, 4>(*d, *p, s);
-    case 1605: return launch_heur<16, 5>(*d, *p, s);
-    case 1606: return launch_heur<16, 6>(*d, *p, s);
-    case 1607: return launch_heur<16, 7>(*d, *p, s);
-    case 3204: return launch_heur<32, 4>(*d, *p, s);
-    case 3205: return launch_heur<32, 5>(*d, *p, s);
-    case 3206: return launch_heur<32, 6>(*d, *p, s);
-    case 3207: return launch_heur<32, 7>(*d, *p, s);
-    case 3208: return launch_heur<32, 8>(*d, *p, s);
+    case 1604: return launch_heur<16, 4>(*d, *p, spl_integer, s);
+    case 1605: return launch_heur<16, 5>(*d, *p, spl_integer, s);
+    case 1606: return launch_heur<16, 6>(*d, *p, spl_integer, s);
+    case 1607: return launch_heur<16, 7>(*d, *p, spl_integer, s);
+    case 3204: return launch_heur<32, 4>(*d, *p, spl_integer, s);
+    case 3205: return launch_heur<32, 5>(*d, *p, spl_integer, s);
+    case 3206: return launch_heur<32, 6>(*d, *p, spl_integer, s);
+    case 3207: return launch_heur<32, 7>(*d, *p, spl_integer, s);
+    case 3208: return launch_heur<32, 8>(*d, *p, spl_integer, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -14,6 +14,7 @@
  * heuristic solutions (with one node per batch this is exactly the
  * reference's Bab_LBGet()).
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <pthread.h>
@@ -32,7 +33,7 @@ hipError_t bqb_launch_csr_build(const BatchDims *d, const BatchPtrs *p, hipStrea
 hipError_t bqb_launch_opbt(const BatchDims *d, const BatchPtrs *p, int use_gtest, hipStream_t s);
 hipError_t bqb_launch_trisep(const BatchDims *d, const BatchPtrs *p, int tri_ineq, hipStream_t s);
 hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStream_t s);
-hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, hipStream_t s);
+hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, int spl_integer, hipStream_t s);
 hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, int kbound, hipStream_t s);
 hipError_t bqb_launch_bundle_step(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
 
@@ -43,6 +44,7 @@ struct bqb_ctx {
     int max_nodes;
     int tile;           /* IPM register tile (P*100 + R) */
     int heur_tile;      /* heuristic tile (P*100 + R) */
+    int spl_int;        /* SP_L integer-valued (|entries| <= 2^20) */
     void *arena;
     size_t arena_bytes;
     hipStream_t stream;
@@ -162,6 +164,9 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     c->max_nodes = max_nodes;
     c->tile = tile;
     c->heur_tile = heur_tile;
+    c->spl_int = 1;
+    for (size_t e = 0; e < (size_t)n0 * n0 && c->spl_int; ++e)
+        if (SP_L[e] != rint(SP_L[e]) || fabs(SP_L[e]) > 1048576.0) c->spl_int = 0;
     BatchDims *d = &c->d;
     d->nodes = max_nodes;
     d->n0 = n0;
@@ -338,7 +343,7 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     switch (what) {
     case 100: return ipm_launch(c, p->L, p->X);
     case 101: return ipm_launch(c, p->L0, p->Xt);
-    case 102: return bqb_check(bqb_launch_heur(d, p, c->heur_tile, s), "heur launch");
+    case 102: return bqb_check(bqb_launch_heur(d, p, c->heur_tile, c->spl_int, s), "heur launch");
     case 103: return bqb_check(bqb_launch_trisep(d, p, c->params.TriIneq, s), "trisep launch");
     case 104: return bqb_check(bqb_launch_sa(d, p, 5, s), "sa5 launch");
     case 105: return bqb_check(bqb_launch_sa(d, p, 7, s), "sa7 launch");

@@ -51,16 +51,16 @@ __device__ __forceinline__ double xg(const double *X, int ld, int a, int b)
     return X[lo + (size_t)hi * ld];
 }
 
-// A chain's permutation: in LDS (16-bit entries) when it fits next to the
-// packed X, otherwise in the node's global scratch.
+// A chain's permutation: in LDS (8-bit entries: vertices < n <= n0 <= 256)
+// when it fits next to the packed X, otherwise in the node's global scratch.
 template <bool LDSP>
 struct PermRef {
-    unsigned short *s;
+    unsigned char *s;
     int *g;
     __device__ __forceinline__ int get(int i) const { return LDSP ? (int)s[i] : g[i]; }
     __device__ __forceinline__ void set(int i, int v) const
     {
-        if (LDSP) s[i] = (unsigned short)v;
+        if (LDSP) s[i] = (unsigned char)v;
         else g[i] = v;
     }
 };
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
         const uint64_t key = bqb_key(rkey, (uint64_t)c);
         uint64_t ctr = 0;
         PermRef<LDSP> perm;
-        perm.s = reinterpret_cast<unsigned short *>(s_X + (XL ? (size_t)ld * (ld + 1) / 2 : 0)) + (size_t)c * ld;
+        perm.s = reinterpret_cast<unsigned char *>(s_X + (XL ? (size_t)d.n0 * (d.n0 + 1) / 2 : 0)) + (size_t)c * d.n0;
         perm.g = p.perm + ((size_t)node * BQB_SA_CHAINS + c) * ld;
         for (int i = 0; i < n; ++i) perm.set(i, i);
         for (int i = n - 1; i > 0; --i) {
@@ -321,9 +321,11 @@ static void launch_sa(const BatchDims *d, const BatchPtrs *p, bool xl, bool lp, 
 
 extern "C" hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStream_t s)
 {
-    const size_t lds_x = sizeof(double) * (size_t)d->ld * (d->ld + 1) / 2;
+    // sized by n0 (every node has n <= n0): the packed triangle of X, then
+    // one byte per vertex and chain
+    const size_t lds_x = sizeof(double) * (size_t)d->n0 * (d->n0 + 1) / 2;
     const int chains = (k == 5) ? d->sa_chains5 : d->sa_chains7;
-    const size_t lds_p = sizeof(unsigned short) * (size_t)chains * d->ld;
+    const size_t lds_p = sizeof(unsigned char) * (size_t)chains * d->n0;
     // X is staged in LDS as a packed triangle while it fits in the 160 KB of
     // a CU (less the kernel's static LDS), else read from the node's slab;
     // the chains' permutations join it in LDS while both fit, otherwise they

@@ -272,13 +272,13 @@ __device__ __forceinline__ int solve_lambda_reg(int k, const double *Q, const do
     return 0;
 }
 
-// KM = the largest bundle the launch may hold: 64 (every case), or 16 when
-// the host knows that no node of the launch has more than 16 elements --
-// then Q and the LDS scratch shrink (20 KB instead of 68 KB) and four
+// KM = the largest bundle the kernel holds: 64 (every case), or 16 -- then
+// Q and the LDS scratch shrink (19 KB instead of 68 KB) and three
 // workgroups fit on a CU.  Both variants do the same arithmetic in the same
-// order.
+// order.  split: the launch is one of a pair, the 16 variant taking the
+// nodes with at most 16 elements and the 64 variant the others.
 template <int KM>
-__global__ __launch_bounds__(BT, (KM == KMAX) ? 2 : 3) void bundle_qp_kernel(BatchDims d, BatchPtrs p)
+__global__ __launch_bounds__(BT, (KM == KMAX) ? 2 : 3) void bundle_qp_kernel(BatchDims d, BatchPtrs p, int split)
 {
     // s_A: the Gram pass's row tiles, the solver's factor, the wave partials
     constexpr int SA = (KM == KMAX) ? KMAX * KMAX : 2048;
@@ -297,6 +297,7 @@ __global__ __launch_bounds__(BT, (KM == KMAX) ? 2 : 3) void bundle_qp_kernel(Bat
     const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
     const int k = s.bundle;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (split && (KM == KMAX ? k <= 16 : k > KM)) return;   // the other launch's node
     if (k > KM || k < 1) {
         if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
         return;
@@ -631,13 +632,19 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
 }  // namespace bqb
 
 // kbound: an upper bound on the bundle size of every node of the launch
-// (0: unknown)
+// (0: unknown -- one launch of the 64 variant).  Above 16 the nodes are split
+// between the two variants by their bundle size (two launches in order).
 extern "C" hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, int kbound, hipStream_t s)
 {
-    if (kbound > 0 && kbound <= 16)
-        hipLaunchKernelGGL(bqb::bundle_qp_kernel<16>, dim3(d->nodes), dim3(bqb::BT), 0, s, *d, *p);
-    else
-        hipLaunchKernelGGL(bqb::bundle_qp_kernel<bqb::KMAX>, dim3(d->nodes), dim3(bqb::BT), 0, s, *d, *p);
+    using namespace bqb;
+    if (kbound <= 0) {
+        hipLaunchKernelGGL(bundle_qp_kernel<KMAX>, dim3(d->nodes), dim3(BT), 0, s, *d, *p, 0);
+    } else if (kbound <= 16) {
+        hipLaunchKernelGGL(bundle_qp_kernel<16>, dim3(d->nodes), dim3(BT), 0, s, *d, *p, 0);
+    } else {
+        hipLaunchKernelGGL(bundle_qp_kernel<16>, dim3(d->nodes), dim3(BT), 0, s, *d, *p, 1);
+        hipLaunchKernelGGL(bundle_qp_kernel<KMAX>, dim3(d->nodes), dim3(BT), 0, s, *d, *p, 1);
+    }
     return hipGetLastError();
 }
 

@@ -151,8 +151,8 @@ def main():
                     help="skip the single-batch comparison after the timed steps (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build bounds on one GPU; --gpus {args.gpus} is not supported "
-              "(the node queue is single-process; sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: bounding runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the node queue is a single process driving one device)", file=sys.stderr)
         sys.exit(2)
 
     from parallel_biqbin_maxcut_amd import api, instances as I

@@ -226,6 +226,52 @@ def test_bundle_qp_matches_oracle_lambda_eta(m, k):
     assert st2.t == pytest.approx(t_o.value)
 
 
+def _qp_setup(ctx, setup, m):
+    for b, (k, G, F, gam) in enumerate(setup):
+        st = ctx.state(b)
+        st.NIneq, st.NPentIneq, st.NHeptaIneq, st.bundle, st.t = m, 0, 0, k, 0.37
+        ctx.set_state(b, st)
+        ctx.put("G", b, np.asfortranarray(G).ravel(order="F"))
+        ctx.put("F", b, F)
+        ctx.put("gamma", b, gam)
+        ctx.put("mask", b, np.array([1], np.int32))
+
+
+def _qp_run(ctx, setup, m, arg):
+    _qp_setup(ctx, setup, m)
+    ctx.run("QP", arg)
+    out = []
+    for b, (k, _, _, _) in enumerate(setup):
+        st = ctx.state(b)
+        out.append((ctx.get("lam", b, np.float64, k), ctx.get("gtest", b, np.float64, m), st.t, st.status))
+    return out
+
+
+@pytest.mark.parametrize("ks,arg", [((6, 14), 16), ((6, 24), 24), ((16, 17), 17)])
+def test_bundle_qp_variants_bitwise_equal(ks, arg):
+    """The QP phase through the full kernel (arg 0), the small-LDS kernel
+    (every bundle <= 16) and the split pair (bundles on both sides of 16)
+    give the same lambda, gamma_test and t bit for bit."""
+    n0 = 20
+    SP = instance(n0, 0.5, "pm1", 2)
+    xf = np.zeros((len(ks), n0 - 1), np.int32)
+    ctx = api.BoundContext(SP, max_nodes=len(ks))
+    basic(ctx, xf, xf)
+    rng = np.random.default_rng(sum(ks))
+    m = 300
+    setup = []
+    for k in ks:
+        G = rng.standard_normal((m, k))
+        F = rng.standard_normal(k) * 3
+        gam = np.abs(rng.standard_normal(m)) * (rng.random(m) < 0.5)
+        setup.append((k, G, F, gam))
+    full = _qp_run(ctx, setup, m, 0)
+    part = _qp_run(ctx, setup, m, arg)
+    for (l0, g0, t0, s0), (l1, g1, t1, s1) in zip(full, part):
+        assert s0 == 0 and s1 == 0
+        assert np.array_equal(l0, l1) and np.array_equal(g0, g1) and t0 == t1
+
+
 def _bound_pair(SP, P, xf, sx, LB, diff, is_root, seeds):
     n0 = SP.shape[0]
     nodes = xf.shape[0]

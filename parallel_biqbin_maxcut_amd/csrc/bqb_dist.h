@@ -45,7 +45,7 @@ struct Lds {
     static constexpr int NMAX = P * R;
     static constexpr int COL = 2 * NMAX;                 // double-buffered pivot column
     static constexpr int PANEL = 2 * 3 * KB * NMAX;      // double-buffered A/X/W GEMM panels
-    static constexpr int TRANS = NMAX * (P + 1);         // symmetrize staging
+    static constexpr int TRANS = (P == 16 ? 2 : 1) * NMAX * (P + 1);   // symmetrize staging
     static constexpr int RED = (P * P / 64) * NMAX;      // row-reduction partials
     static constexpr int UNION = PANEL > TRANS ? (PANEL > RED ? PANEL : RED) : (TRANS > RED ? TRANS : RED);
 };
@@ -394,24 +394,44 @@ __device__ __forceinline__ bool chol_lower(double (&A)[R][R], int n, double *col
 }
 
 // ---------------------------------------------------------------- symmetrize
-// A <- (A + A')/2 using LDS staging of one register column block per round.
-// Element (r,b) with b < r takes the partner's value, which the partner
-// already averaged in round b; (r,b) with b >= r averages the two originals.
+// A <- (A + A')/2 using LDS staging of register column blocks: two, r and
+// r + 1, per round on the 16 x 16 grid (T holds 2 NMAX (P+1) doubles), one
+// on the 32 x 32 grid (NMAX (P+1)).  Element (r,b) with b < r takes the
+// partner's value, which the partner already averaged in an earlier round;
+// (r,b) with b >= r averages the two originals.  With two blocks per round,
+// row block r + 1 does the same except at b = r: that partner is averaged
+// in this round, so both sides average the originals (IEEE addition
+// commutes: the same value on both sides).
 template <int P, int R>
 __device__ __forceinline__ void symmetrize(double (&A)[R][R], double *T, const Thr &t)
 {
     constexpr int LD = P + 1;
-    static_for<0, R>([&](auto rc) {
-        constexpr int r = decltype(rc)::value;
+    constexpr int TS = P * R * LD;
+    constexpr int STEP = (P == 16) ? 2 : 1;
+    static_for<0, (R + STEP - 1) / STEP>([&](auto rc) {
+        constexpr int r0 = STEP * decltype(rc)::value;
+        constexpr int r1 = (STEP == 2) ? r0 + 1 : R;
         __syncthreads();
 #pragma unroll
-        for (int a = 0; a < R; ++a) T[(t.tr + P * a) * LD + t.tc] = A[a][r];
+        for (int a = 0; a < R; ++a) T[(t.tr + P * a) * LD + t.tc] = A[a][r0];
+        if constexpr (r1 < R) {
+#pragma unroll
+            for (int a = 0; a < R; ++a) T[TS + (t.tr + P * a) * LD + t.tc] = A[a][r1];
+        }
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < R; ++b) {
             const double v = T[(t.tc + P * b) * LD + t.tr];
-            if (b < r) A[r][b] = v;
-            else A[r][b] = 0.5 * (A[r][b] + v);
+            if (b < r0) A[r0][b] = v;
+            else A[r0][b] = 0.5 * (A[r0][b] + v);
+        }
+        if constexpr (r1 < R) {
+#pragma unroll
+            for (int b = 0; b < R; ++b) {
+                const double v = T[TS + (t.tc + P * b) * LD + t.tr];
+                if (b < r0) A[r1][b] = v;
+                else A[r1][b] = 0.5 * (A[r1][b] + v);
+            }
         }
     });
     __syncthreads();

@@ -15,7 +15,7 @@ __global__ __launch_bounds__(P * P) void dist_selftest_kernel(int op, int n, int
 {
     constexpr int NMAX = P * R;
     __shared__ double s_col[2 * NMAX];
-    __shared__ double s_T[NMAX * (P + 1)];
+    __shared__ double s_T[Lds<P, R, 8>::TRANS];
     const Thr t = thr_id<P>();
     const Slab si = make_slab<P>(in, ld, t), so = make_slab<P>(out, ld, t);
     double A[R][R];

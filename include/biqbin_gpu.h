@@ -156,7 +156,7 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
 
 /* ---- batch: interior-point solve of the basic SDP ---------------------- */
 /* Solves max <L_b, X_b> s.t. diag(X_b) = e, X_b psd for `nodes` independent
- * matrices of sizes n[b] (<= 160).  L: the matrices concatenated, each
+ * matrices of sizes n[b] (<= 256).  L: the matrices concatenated, each
  * column-major n[b] x n[b]; X (same layout) receives the primal matrices;
  * phi[b] = e'y (the dual value, a valid upper bound).  iters/trials/status/
  * y_out (concatenated, length sum n[b]) / alpha_trace (nodes x 2*trace_cap)
@@ -167,6 +167,23 @@ int bqb_ipm_ld_for(int nmax);
 
 int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
                   int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap);
+
+/* No counterpart in the reference.  The batched IPM of bqb_ipm_batch for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): every pointer is
+ * device memory owned by the caller -- n[nodes] the sizes (each <= nmax),
+ * L the matrices concatenated as in bqb_ipm_batch, X (same layout) the
+ * primal matrices, phi/iters/status[nodes] -- and the work memory comes
+ * from one library-owned device array of BQB_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends.  Calls needing more (bqb_ipm_noalloc_bytes)
+ * are rejected with -1.  Work is enqueued on `stream` (a hipStream_t, NULL
+ * for the default stream) and not waited for.  The function calls no
+ * hipMalloc/hipFree.  All calls on one GPU share its array: calls that
+ * overlap on one GPU race. */
+#define BQB_NOALLOC_SCRATCH_BYTES (64u << 20)
+size_t bqb_ipm_noalloc_bytes(int nodes, int nmax);
+int bqb_ipm_batch_noalloc(int nodes, int nmax, const int *n, const double *L, double *X, double *phi,
+                          int *iters, int *status, void *stream);
 
 /* Tuning hook (no counterpart in the reference): force the IPM register
  * tile (P*100 + R, one of 1604-1607, 3204, 3205) for contexts and batches

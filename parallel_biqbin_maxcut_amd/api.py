@@ -19,6 +19,11 @@ def declare(L):
     L.bqb_set_ipm_tile.argtypes = [C.c_int]
     L.bqb_ipm_ld_for.argtypes = [C.c_int]
     L.bqb_ipm_ld_for.restype = C.c_int
+    vp = C.c_void_p
+    L.bqb_ipm_noalloc_bytes.argtypes = [C.c_int, C.c_int]
+    L.bqb_ipm_noalloc_bytes.restype = C.c_size_t
+    L.bqb_ipm_batch_noalloc.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    L.bqb_ipm_batch_noalloc.restype = C.c_int
     _declare_ctx(L)
 
 

@@ -45,6 +45,7 @@ int bqb_init(int device)
     if (device < 0) device = 0;
     if (device >= count) { bqb_set_error("device %d out of range (%d devices)", device, count); return -1; }
     if (bqb_check(hipSetDevice(device), "hipSetDevice")) return -1;
+    if (bqb_noalloc_setup()) return -1;
     g_device = device;
     return 0;
 }

@@ -18,6 +18,8 @@ void bqb_buf_free(bqb_buf *b);
 int bqb_hbuf_reserve(bqb_buf *b, size_t bytes);
 void bqb_hbuf_free(bqb_buf *b);
 int bqb_ipm_ld_for(int nmax);
+/* device address of the no-allocation scratch array on the current device */
+int bqb_noalloc_setup(void);
 
 #include "../../include/biqbin_gpu.h"
 

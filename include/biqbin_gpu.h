@@ -64,6 +64,20 @@ typedef struct BiqBinParameters {
 #define BQB_ERR_SIZE 11
 #define BQB_ERR_CUTS_FULL 12
 
+/* ---- per-node event flags (stats[8] of the batch API) ------------------ */
+/* The bundle reached the context's capacity: the node ends as a give-up
+ * with its bound at that moment (f + fixedvalue, a valid bound), where the
+ * reference aborts the run (src/bounding.c:312-315). */
+#define BQB_FLAG_BUNDLE_FULL 1
+/* A separation produced more triangle / pentagonal / heptagonal cuts than
+ * the context holds; the surplus was dropped (the bound stays valid, but
+ * its trajectory departs from the reference's). */
+#define BQB_FLAG_TRI_TRUNC 2
+#define BQB_FLAG_PENT_TRUNC 4
+#define BQB_FLAG_HEPT_TRUNC 8
+/* ints per node in the stats arrays of the batch API */
+#define BQB_NSTATS 9
+
 /* ---- library management ---------------------------------------------- */
 int bqb_init(int device);                 /* select the HIP device (default 0) */
 const char *bqb_last_error(void);
@@ -144,6 +158,8 @@ typedef struct bqb_solve_stats {
     int max_queue;
     double seconds;       /* wall time */
     double bound_ms;      /* time inside the batched bound (device events) */
+    long long bundle_full;     /* nodes given up because their bundle was full */
+    long long cuts_truncated;  /* nodes whose separation overflowed a cut capacity */
 } bqb_solve_stats;
 
 /* Single-process branch and bound on one GPU; replaces the MPI
@@ -153,6 +169,11 @@ typedef struct bqb_solve_stats {
  * every vertex but the last, which is on side 0).  Returns 0 on success. */
 int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
               int *sol, bqb_solve_stats *stats);
+/* The same with a bundle capacity per node (<= 0: the default 64); a node
+ * whose bundle fills ends as a give-up (BQB_FLAG_BUNDLE_FULL) and is
+ * branched. */
+int bqb_solve_cap(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
+                  int max_bundle, int *sol, bqb_solve_stats *stats);
 
 /* ---- batch: interior-point solve of the basic SDP ---------------------- */
 /* Solves max <L_b, X_b> s.t. diag(X_b) = e, X_b psd for `nodes` independent
@@ -213,8 +234,9 @@ double bqb_ctx_last_ms(const bqb_ctx *c);
  * rank == 0 semantics, seeds[b] the node's random stream.  Outputs per node:
  * bound (f + fixedvalue), fracsol (n0-1), xbest (n0-1, best heuristic cut;
  * valid if improved), LB_out (LB raised by the node's own heuristic),
- * diff_out, status, and 8 stats ints (prune, giveup, outer iterations, IPM
- * calls, IPM iterations, max cuts, max bundle size, n).  Any output pointer
+ * diff_out, status, and BQB_NSTATS stats ints (prune, giveup, outer
+ * iterations, IPM calls, IPM iterations, max cuts, max bundle size, n,
+ * BQB_FLAG_* events).  Any output pointer
  * may be NULL.  A node's results depend only on its own inputs. */
 int bqb_bound_batch(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
                     const double *diff, const int *is_root, const uint64_t *seeds, double *bound,

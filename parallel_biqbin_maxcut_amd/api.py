@@ -144,7 +144,7 @@ class NodeState(C.Structure):
         "n", "status", "active", "prune", "giveup", "done", "count", "bdl_iter", "NIneq",
         "NPentIneq", "NHeptaIneq", "bundle", "sep_tri", "sep_pent", "sep_hept", "heur_idx",
         "improved", "ipm_calls", "ipm_iters", "max_cuts", "max_bundle_seen", "is_root", "serious",
-        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase")] + [(f, C.c_double) for f in (
+        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags")] + [(f, C.c_double) for f in (
         "fixedvalue", "f", "f_test", "t", "bound", "basic_bound", "oldf", "LB", "diff", "viol3",
         "viol5", "gap", "del_", "lmax")] + [("seed", C.c_uint64)]
 
@@ -181,10 +181,13 @@ def _arr(ptr, count, dtype, shape=None):
 class SolveStats(C.Structure):
     _fields_ = [("best", C.c_double), ("root_bound", C.c_double), ("nodes", C.c_longlong),
                 ("batches", C.c_longlong), ("status", C.c_int), ("stopped", C.c_int),
-                ("max_queue", C.c_int), ("seconds", C.c_double), ("bound_ms", C.c_double)]
+                ("max_queue", C.c_int), ("seconds", C.c_double), ("bound_ms", C.c_double),
+                ("bundle_full", C.c_longlong), ("cuts_truncated", C.c_longlong)]
 
 
 NMAX = 1024
+NSTATS = 9   # ints per node in the stats arrays (BQB_NSTATS)
+FLAG_BUNDLE_FULL, FLAG_TRI_TRUNC, FLAG_PENT_TRUNC, FLAG_HEPT_TRUNC = 1, 2, 4, 8
 
 
 class Problem(C.Structure):
@@ -210,6 +213,8 @@ def _declare_host(L):
     L.bqb_evaluate_solution.restype = C.c_double
     L.bqb_solve.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, N.iptr,
                             C.POINTER(SolveStats)]
+    L.bqb_solve_cap.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, C.c_int,
+                                N.iptr, C.POINTER(SolveStats)]
     for f in ("Evaluate", "SDPbound"):
         getattr(L, f).argtypes = [C.POINTER(BabNode), C.POINTER(Problem), C.POINTER(Problem), C.c_int]
         getattr(L, f).restype = C.c_double
@@ -267,15 +272,16 @@ def evaluate_solution(SP_L, sol):
     return N.lib().bqb_evaluate_solution(SP_L.shape[0], _d(SP_L), _i(s))
 
 
-def solve(SP_L, params=None, seed=2020, batch=64):
-    """Branch and bound on the GPU (bqb_solve).  Returns (stats, solution)
-    with the solution over the first n0-1 vertices (the last is on side 0)."""
+def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0):
+    """Branch and bound on the GPU (bqb_solve / bqb_solve_cap).  Returns
+    (stats, solution) with the solution over the first n0-1 vertices (the
+    last is on side 0)."""
     SP_L = np.asfortranarray(SP_L, dtype=np.float64)
     n0 = SP_L.shape[0]
     p = params if params is not None else make_params()
     sol = np.zeros(n0 - 1, dtype=np.int32)
     st = SolveStats()
-    N.check(N.lib().bqb_solve(n0, _d(SP_L), C.byref(p), seed, batch, _i(sol), C.byref(st)))
+    N.check(N.lib().bqb_solve_cap(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, _i(sol), C.byref(st)))
     return st, sol
 
 
@@ -377,7 +383,7 @@ class BoundContext:
         B = self.n0 - 1
         out = dict(bound=np.zeros(nodes), fracsol=np.zeros((nodes, B)), xbest=np.zeros((nodes, B), np.int32),
                    LB=np.zeros(nodes), improved=np.zeros(nodes, np.int32), diff=np.zeros(nodes),
-                   status=np.zeros(nodes, np.int32), stats=np.zeros((nodes, 8), np.int32))
+                   status=np.zeros(nodes, np.int32), stats=np.zeros((nodes, NSTATS), np.int32))
         args = (nodes, _i(xf), _i(sx), _d(lb), _d(df), _i(rt), sd.ctypes.data_as(C.POINTER(C.c_uint64)),
                 _d(out["bound"]), _d(out["fracsol"]), _i(out["xbest"]), _d(out["LB"]), _i(out["improved"]),
                 _d(out["diff"]), _i(out["status"]), _i(out["stats"]))

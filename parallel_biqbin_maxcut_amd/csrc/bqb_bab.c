@@ -259,11 +259,24 @@ static double now_s(void)
 int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
               int *sol, bqb_solve_stats *out)
 {
+    return bqb_solve_cap(n0, SP_L, params, seed, batch, 0, sol, out);
+}
+
+/* Counts a finished node's events in the run statistics. */
+static void count_flags(bqb_solve_stats *out, int flags)
+{
+    if (flags & BQB_FLAG_BUNDLE_FULL) out->bundle_full += 1;
+    if (flags & (BQB_FLAG_TRI_TRUNC | BQB_FLAG_PENT_TRUNC | BQB_FLAG_HEPT_TRUNC)) out->cuts_truncated += 1;
+}
+
+int bqb_solve_cap(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
+                  int max_bundle, int *sol, bqb_solve_stats *out)
+{
     memset(out, 0, sizeof *out);
     const double t0 = now_s();
     if (batch < 1) batch = 1;
     const int B = n0 - 1;
-    bqb_ctx *ctx = bqb_ctx_create(n0, SP_L, params, batch, 0, 0, 0, 0);
+    bqb_ctx *ctx = bqb_ctx_create(n0, SP_L, params, batch, max_bundle, 0, 0, 0);
     if (!ctx) return -1;
     qheap h = {0};
     int rc = -1;
@@ -283,7 +296,7 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
         qnode *root = node_new(NULL, B, next_id++);
         if (!root) goto done;
         double bound, lbo, dfo;
-        int imp, st, stats[8];
+        int imp, st, stats[BQB_NSTATS];
         const double LB0 = LB, df0 = 0.0;
         const int is_root = 1;
         const uint64_t sd = bqb_key(seed, root->id);
@@ -295,6 +308,7 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
         out->bound_ms += bqb_ctx_last_ms(ctx);
         out->nodes = 1;
         out->batches = 1;
+        count_flags(out, stats[8]);
         if (st) {
             out->status = st;
             bqb_set_error("root node bound failed with status %d", st);
@@ -340,13 +354,14 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
         out->batches += 1;
         long long t;
         double bound, lbo, dfo;
-        int imp, st;
+        int imp, st, sts[BQB_NSTATS];
         int fail = 0;
-        while (bqb_ctx_pop(ctx, &t, &bound, frac, xb, &lbo, &imp, &dfo, &st, NULL)) {
+        while (bqb_ctx_pop(ctx, &t, &bound, frac, xb, &lbo, &imp, &dfo, &st, sts)) {
             qnode *x = inflight[t];
             inflight[t] = NULL;
             tag_free[ntag++] = (int)t;
             out->nodes += 1;
+            count_flags(out, sts[8]);
             if (st && !out->status) {
                 out->status = st;
                 bqb_set_error("node bound failed with status %d", st);

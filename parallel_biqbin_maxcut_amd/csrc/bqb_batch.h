@@ -43,6 +43,7 @@ typedef struct NodeState {
     int stage;          /* slot life cycle: BQB_STAGE_* */
     int bdl_left;       /* bundle iterations left in the current outer round */
     int rphase;         /* outer round: 0 to begin, 1 bundle iterations, 2 round end */
+    int flags;          /* BQB_FLAG_* events of the node (reported in stats[8]) */
     double fixedvalue, f, f_test, t, bound, basic_bound, oldf;
     double LB, diff, viol3, viol5, gap, del, lmax;
     uint64_t seed;

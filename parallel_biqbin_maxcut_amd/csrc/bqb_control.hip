@@ -289,7 +289,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
     }
 
     if (phase == PH_OUTER_BEGIN) {
-        // bounding.c:170-174; the bundle length of outer round c is
+        // bounding.c:170-174; the bundle length of outer round c >= 2 is
         // init_bundle_iter + floor(c/2), at most max_bundle_iter
         // (bounding.c:323-325 add c % 2 after every round)
         if (threadIdx.x == 0) {
@@ -298,7 +298,10 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
                 const int bdl = P.init_bundle_iter + c / 2;
                 sp->count = c;
                 sp->oldf = s.f;
-                sp->bdl_iter = bdl < P.max_bundle_iter ? bdl : P.max_bundle_iter;
+                // round 1 runs init_bundle_iter as given; the cap applies
+                // from the first increment on (bounding.c:50, 323-325)
+                sp->bdl_iter = (c == 1) ? P.init_bundle_iter
+                                        : (bdl < P.max_bundle_iter ? bdl : P.max_bundle_iter);
                 sp->bdl_left = sp->bdl_iter;
                 sp->rphase = sp->bdl_left > 0 ? 1 : 2;
             }
@@ -403,7 +406,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         if (!s.active) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
         const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
         if (s.bundle >= d.max_bundle) {
-            if (threadIdx.x == 0) { sp->status = ERR_BUNDLE_FULL; sp->active = 0; p.mask[node] = 0; }
+            if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node);
             return;
         }
         double *G = p.G + (size_t)node * d.max_bundle * d.max_m;

@@ -13,6 +13,19 @@
 
 namespace bqb {
 
+// The node's bundle is full: it leaves SDPbound as a give-up with the bound
+// of its current stability center, f + fixedvalue (a valid bound for any
+// multipliers gamma >= 0), and the event is flagged.  One thread calls it.
+__device__ __forceinline__ void bundle_full_giveup(NodeState *sp, const NodeState &s, int *mask)
+{
+    sp->giveup = 1;
+    sp->done = 1;
+    sp->active = 0;
+    sp->bound = s.f + s.fixedvalue;
+    sp->flags = s.flags | BQB_FLAG_BUNDLE_FULL;
+    *mask = 0;
+}
+
 __device__ __forceinline__ int tri_e(int type, int a)
 {
     // type 2 flips e[2], type 3 flips e[1], type 4 flips e[0]

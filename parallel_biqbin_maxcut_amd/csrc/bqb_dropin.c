@@ -140,7 +140,7 @@ double SDPbound(BqbBabNode *node, BqbProblem *SP, BqbProblem *PP, int rank)
     const int is_root = rank == 0;
     const uint64_t seed = bqb_key(2020, g_drop.evals++);
     double bound, LB_out, diff_out;
-    int improved, status, stats[8];
+    int improved, status, stats[BQB_NSTATS];
     int *xbest = malloc(sizeof(int) * B);
     if (!xbest) { fprintf(stderr, "Error: memory allocation in SDPbound\n"); exit(10); }
     if (bqb_bound_batch(c, 1, node->xfixed, node->solX, &LB, dp, &is_root, &seed, &bound,

@@ -300,8 +300,10 @@ __global__ __launch_bounds__(ST) void sa_kernel(BatchDims d, BatchPtrs p)
         added += s_qn;
         __syncthreads();
     }
-    if (kept + added > maxc) added = maxc - kept;
+    const bool trunc = kept + added > maxc;   // more new cuts than the context holds
+    if (trunc) added = maxc - kept;
     if (threadIdx.x == 0) {
+        if (trunc) sp->flags = s.flags | ((K == 5) ? BQB_FLAG_PENT_TRUNC : BQB_FLAG_HEPT_TRUNC);
         if (K == 5) sp->NPentIneq = kept + added;
         else sp->NHeptaIneq = kept + added;
         sp->cuts_changed = 1;

@@ -587,8 +587,10 @@ int bqb_ctx_advance(bqb_ctx *c)
         in_loop = 1;
         int left = s->bdl_left;
         if (s->rphase == 0) {
+            /* the bundle length of the round it begins (PH_OUTER_BEGIN) */
             left = P->init_bundle_iter + (s->count + 1) / 2;
-            if (left > P->max_bundle_iter) left = P->max_bundle_iter;
+            if (s->count == 0) left = P->init_bundle_iter;
+            else if (left > P->max_bundle_iter) left = P->max_bundle_iter;
         } else if (s->rphase == 2) {
             left = 0;
         }
@@ -681,6 +683,7 @@ int bqb_ctx_pop(bqb_ctx *c, long long *tag, double *bound, double *fracsol, int 
     if (stats) {
         stats[0] = s->prune; stats[1] = s->giveup; stats[2] = s->count; stats[3] = s->ipm_calls;
         stats[4] = s->ipm_iters; stats[5] = s->max_cuts; stats[6] = s->max_bundle_seen; stats[7] = s->n;
+        stats[8] = s->flags;
     }
     c->done_head = (c->done_head + 1) % c->done_cap;
     --c->ndone;
@@ -730,9 +733,10 @@ int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, doub
         if (diff_out) diff_out[i] = s->diff;
         if (status) status[i] = s->status;
         if (stats) {
-            int *o = stats + 8 * i;
+            int *o = stats + BQB_NSTATS * i;
             o[0] = s->prune; o[1] = s->giveup; o[2] = s->count; o[3] = s->ipm_calls;
             o[4] = s->ipm_iters; o[5] = s->max_cuts; o[6] = s->max_bundle_seen; o[7] = s->n;
+            o[8] = s->flags;
         }
     }
     return 0;
@@ -757,6 +761,9 @@ struct stream_job {
     double *diff_out;
     int *status, *stats;
 };
+
+/* error of a context that stopped because another context failed */
+#define STREAM_STOPPED "stopped: another context failed"
 
 static int stream_take(struct stream_job *j)
 {
@@ -796,7 +803,7 @@ static int stream_run(bqb_ctx *c, struct stream_job *j)
         if (bqb_ctx_advance(c)) goto out;
         long long t;
         double bd, lbo, dfo;
-        int imp, st, sts[8];
+        int imp, st, sts[BQB_NSTATS];
         while (bqb_ctx_pop(c, &t, &bd, frac, xb, &lbo, &imp, &dfo, &st, sts)) {
             if (j->bound) j->bound[t] = bd;
             if (j->fracsol) memcpy(j->fracsol + t * B, frac, sizeof(double) * B);
@@ -805,9 +812,9 @@ static int stream_run(bqb_ctx *c, struct stream_job *j)
             if (j->improved) j->improved[t] = imp;
             if (j->diff_out) j->diff_out[t] = dfo;
             if (j->status) j->status[t] = st;
-            if (j->stats) memcpy(j->stats + 8 * t, sts, sizeof sts);
+            if (j->stats) memcpy(j->stats + BQB_NSTATS * t, sts, sizeof sts);
         }
-        if (stream_failed(j)) { bqb_set_error("stopped: another context failed"); goto out; }
+        if (stream_failed(j)) { bqb_set_error(STREAM_STOPPED); goto out; }
     }
     rc = 0;
 out:
@@ -869,6 +876,11 @@ int bqb_bound_stream_multi(bqb_ctx *const *ctx, int nctx, int count, const int *
     }
     hipEventRecord(ctx[0]->ev0, ctx[0]->stream);
     int rc = stream_run(ctx[0], &j);
+    /* the first real error wins: a context that only stopped because
+     * another one failed reports the sentinel, not the cause */
+    char err0[512];
+    snprintf(err0, sizeof err0, "%s", rc ? bqb_last_error() : "");
+    int real = rc && strcmp(err0, STREAM_STOPPED) != 0;
     if (rc) {
         pthread_mutex_lock(&j.mu);
         j.failed = 1;
@@ -876,8 +888,12 @@ int bqb_bound_stream_multi(bqb_ctx *const *ctx, int nctx, int count, const int *
     }
     for (int i = 1; i <= started; ++i) {
         pthread_join(th[i], NULL);
-        if (w[i].rc && !rc) { rc = -1; bqb_set_error("%s", w[i].err); }
+        if (!w[i].rc) continue;
+        if (!real && strcmp(w[i].err, STREAM_STOPPED) != 0) { snprintf(err0, sizeof err0, "%s", w[i].err); real = 1; }
+        else if (!rc) snprintf(err0, sizeof err0, "%s", w[i].err);
+        rc = -1;
     }
+    if (rc) bqb_set_error("%s", err0);
     if (!rc && started < nctx - 1) { rc = -1; bqb_set_error("pthread_create failed"); }
     pthread_mutex_destroy(&j.mu);
     if (rc) return rc;

@@ -309,7 +309,8 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
         added += s_qn;
         __syncthreads();
     }
-    if (kept + added > d.max_tri) added = d.max_tri - kept;
+    const bool trunc = kept + added > d.max_tri;   // more new cuts than the context holds
+    if (trunc) added = d.max_tri - kept;
     // the list itself (values in scan-replay order) for inspection
     TriCut *lst = p.tri_lst + (size_t)node * d.tri_list;
     double *lv = p.tri_lval + (size_t)node * d.tri_list;
@@ -325,6 +326,7 @@ __global__ __launch_bounds__(TT) void trisep_kernel(BatchDims d, BatchPtrs p)
         sp->off_pent = s.NIneq;
         sp->off_hept = s.NIneq + s.NPentIneq;
         sp->NIneq = kept + added;
+        if (trunc) sp->flags = s.flags | BQB_FLAG_TRI_TRUNC;
         sp->viol3 = maxall;
         sp->cuts_changed = 1;
     }

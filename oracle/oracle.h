@@ -217,9 +217,16 @@ void orc_params_default(orc_params *p);
  *   1 bundle step   (count, f_test, f, t, bundle, m, serious)
  *   2 QP            (count, t, lambda_0, sum dgamma^2, bundle, m, 0)
  *   3 outer round   (count, NIneq, NPentIneq, NHeptaIneq, viol3, LB, bound)
- *   4 heuristic     (count, LB, value, 0, 0, 0, 0)                       */
+ *   4 heuristic     (count, LB, value, 0, 0, 0, 0)
+ *   5 step decision (count, del, f before the step, f_test, max lambda, k, serious)
+ *   6 bundle element (count, i, lambda_i, max lambda, k, serious, 0), one
+ *     row per element i < k of the step, before the purge                */
 void orc_ws_set_trace(orc_ws *w, double *buf, int cap);
 int orc_ws_trace_n(const orc_ws *w);
+
+/* Test helper: the triangle update on a given X and cut list (orc_bound.c). */
+double orc_update_triangles_on(const double *X, int n, const orc_params *p, orc_tri *cuts, int *ncuts,
+                               int max_tri, const double *y, orc_tri *list, int *list_size);
 
 /* Test helper: the heuristic of one node on a given X (see orc_heur.c). */
 double orc_heuristic_on(const double *SP_L, int n0, const int *xfixed, const int *solX,

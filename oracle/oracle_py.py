@@ -65,64 +65,97 @@ class BabOut(C.Structure):
 
 
 def lib():
+    """The oracle library in use: liboracle.so, or the variant selected by
+    variant() (tests that show their checks detect a changed constant)."""
     global _LIB
+    if _VARIANT is not None:
+        return _VARIANT
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
-        L = C.CDLL(path)
-        L.orc_ipm_mc_pk_ex.argtypes = [dptr, C.c_int, dptr, dptr, dptr, C.POINTER(IpmStats), dptr, C.c_int]
-        L.orc_op_B.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, dptr, dptr]
-        L.orc_op_Bt.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, dptr, dptr]
-        L.orc_get_violated_triangles.argtypes = [dptr, C.c_int, C.c_double, C.c_int, C.POINTER(Tri), iptr]
-        L.orc_get_violated_triangles.restype = C.c_double
-        L.orc_qap_sa.argtypes = [iptr, C.c_int, dptr, C.c_int, iptr, C.c_uint64, iptr]
-        L.orc_qap_sa.restype = C.c_double
-        L.orc_H_pent.argtypes = [C.c_int]
-        L.orc_H_pent.restype = iptr
-        L.orc_H_hept.argtypes = [C.c_int]
-        L.orc_H_hept.restype = iptr
-        L.orc_get_violated_pent.argtypes = [dptr, C.c_int, C.c_int, C.c_int, C.POINTER(Pent), iptr, C.c_uint64, iptr]
-        L.orc_get_violated_pent.restype = C.c_double
-        L.orc_get_violated_hept.argtypes = [dptr, C.c_int, C.c_int, C.c_int, C.POINTER(Hept), iptr, C.c_uint64, iptr]
-        L.orc_get_violated_hept.restype = C.c_double
-        L.orc_mc_1opt.argtypes = [iptr, dptr, C.c_int]
-        L.orc_mc_1opt.restype = C.c_double
-        L.orc_evaluate_solution.argtypes = [dptr, C.c_int, iptr]
-        L.orc_evaluate_solution.restype = C.c_double
-        L.orc_create_subproblem.argtypes = [dptr, C.c_int, iptr, iptr, dptr]
-        L.orc_create_subproblem.restype = C.c_int
-        L.orc_get_fixed_value.argtypes = [dptr, C.c_int, iptr, iptr]
-        L.orc_get_fixed_value.restype = C.c_double
-        L.orc_build_L_from_adj.argtypes = [dptr, C.c_int, dptr]
-        L.orc_solve_lambda.argtypes = [C.c_int, dptr, dptr, dptr]
-        L.orc_solve_lambda.restype = C.c_int
-        L.orc_lambda_eta.argtypes = [C.c_int, C.c_int, dptr, dptr, C.c_int, dptr, dptr, dptr, dptr, dptr]
-        L.orc_lambda_eta.restype = C.c_int
-        L.orc_fct_eval.argtypes = [dptr, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
-                                   dptr, dptr, dptr, C.POINTER(IpmStats)]
-        L.orc_fct_eval.restype = C.c_double
-        L.orc_ws_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Params)]
-        L.orc_ws_create.restype = C.c_void_p
-        L.orc_ws_destroy.argtypes = [C.c_void_p]
-        L.orc_sdpbound.argtypes = [C.c_void_p, dptr, C.c_int, C.POINTER(NodeIn), C.POINTER(NodeOut), dptr, iptr]
-        L.orc_bab_solve.argtypes = [dptr, C.c_int, C.POINTER(Params), C.c_uint64, C.c_int, iptr, C.POINTER(BabOut)]
-        L.orc_params_default.argtypes = [C.POINTER(Params)]
-        L.orc_sdpbound_many.argtypes = [dptr, C.c_int, C.POINTER(Params), C.c_int, iptr, iptr, dptr, dptr,
-                                        C.POINTER(C.c_uint64), C.c_int, dptr, iptr]
-        L.orc_key.argtypes = [C.c_uint64, C.c_uint64]
-        L.orc_key.restype = C.c_uint64
-        L.orc_draw.argtypes = [C.c_uint64, C.c_uint64]
-        L.orc_draw.restype = C.c_uint64
-        L.orc_heuristic_on.argtypes = [dptr, C.c_int, iptr, iptr, dptr, dptr, C.c_int, C.c_uint64, iptr, iptr]
-        L.orc_heuristic_on.restype = C.c_double
-        L.orc_ws_set_trace.argtypes = [C.c_void_p, dptr, C.c_int]
-        L.orc_ws_trace_n.argtypes = [C.c_void_p]
-        L.orc_ws_trace_n.restype = C.c_int
-        L.orc_chol_upper.argtypes = [dptr, C.c_int]
-        L.orc_chol_upper.restype = C.c_int
-        _LIB = L
+        _LIB = load(path)
     return _LIB
+
+
+_VARIANT = None
+
+
+class variant:
+    """Context manager: every wrapper of this module calls the library at
+    `path` (an oracle built with other constants) inside the block."""
+
+    def __init__(self, path):
+        self.L = load(path)
+
+    def __enter__(self):
+        global _VARIANT
+        self.prev, _VARIANT = _VARIANT, self.L
+        return self.L
+
+    def __exit__(self, *exc):
+        global _VARIANT
+        _VARIANT = self.prev
+        return False
+
+
+def load(path):
+    """Load an oracle shared library and declare its signatures."""
+    L = C.CDLL(path)
+    L.orc_ipm_mc_pk_ex.argtypes = [dptr, C.c_int, dptr, dptr, dptr, C.POINTER(IpmStats), dptr, C.c_int]
+    L.orc_op_B.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, dptr, dptr]
+    L.orc_op_Bt.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, dptr, dptr]
+    L.orc_get_violated_triangles.argtypes = [dptr, C.c_int, C.c_double, C.c_int, C.POINTER(Tri), iptr]
+    L.orc_get_violated_triangles.restype = C.c_double
+    L.orc_qap_sa.argtypes = [iptr, C.c_int, dptr, C.c_int, iptr, C.c_uint64, iptr]
+    L.orc_qap_sa.restype = C.c_double
+    L.orc_H_pent.argtypes = [C.c_int]
+    L.orc_H_pent.restype = iptr
+    L.orc_H_hept.argtypes = [C.c_int]
+    L.orc_H_hept.restype = iptr
+    L.orc_get_violated_pent.argtypes = [dptr, C.c_int, C.c_int, C.c_int, C.POINTER(Pent), iptr, C.c_uint64, iptr]
+    L.orc_get_violated_pent.restype = C.c_double
+    L.orc_get_violated_hept.argtypes = [dptr, C.c_int, C.c_int, C.c_int, C.POINTER(Hept), iptr, C.c_uint64, iptr]
+    L.orc_get_violated_hept.restype = C.c_double
+    L.orc_mc_1opt.argtypes = [iptr, dptr, C.c_int]
+    L.orc_mc_1opt.restype = C.c_double
+    L.orc_evaluate_solution.argtypes = [dptr, C.c_int, iptr]
+    L.orc_evaluate_solution.restype = C.c_double
+    L.orc_create_subproblem.argtypes = [dptr, C.c_int, iptr, iptr, dptr]
+    L.orc_create_subproblem.restype = C.c_int
+    L.orc_get_fixed_value.argtypes = [dptr, C.c_int, iptr, iptr]
+    L.orc_get_fixed_value.restype = C.c_double
+    L.orc_build_L_from_adj.argtypes = [dptr, C.c_int, dptr]
+    L.orc_solve_lambda.argtypes = [C.c_int, dptr, dptr, dptr]
+    L.orc_solve_lambda.restype = C.c_int
+    L.orc_lambda_eta.argtypes = [C.c_int, C.c_int, dptr, dptr, C.c_int, dptr, dptr, dptr, dptr, dptr]
+    L.orc_lambda_eta.restype = C.c_int
+    L.orc_fct_eval.argtypes = [dptr, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                               dptr, dptr, dptr, C.POINTER(IpmStats)]
+    L.orc_fct_eval.restype = C.c_double
+    L.orc_ws_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Params)]
+    L.orc_ws_create.restype = C.c_void_p
+    L.orc_ws_destroy.argtypes = [C.c_void_p]
+    L.orc_sdpbound.argtypes = [C.c_void_p, dptr, C.c_int, C.POINTER(NodeIn), C.POINTER(NodeOut), dptr, iptr]
+    L.orc_bab_solve.argtypes = [dptr, C.c_int, C.POINTER(Params), C.c_uint64, C.c_int, iptr, C.POINTER(BabOut)]
+    L.orc_params_default.argtypes = [C.POINTER(Params)]
+    L.orc_sdpbound_many.argtypes = [dptr, C.c_int, C.POINTER(Params), C.c_int, iptr, iptr, dptr, dptr,
+                                    C.POINTER(C.c_uint64), C.c_int, dptr, iptr]
+    L.orc_key.argtypes = [C.c_uint64, C.c_uint64]
+    L.orc_key.restype = C.c_uint64
+    L.orc_draw.argtypes = [C.c_uint64, C.c_uint64]
+    L.orc_draw.restype = C.c_uint64
+    L.orc_heuristic_on.argtypes = [dptr, C.c_int, iptr, iptr, dptr, dptr, C.c_int, C.c_uint64, iptr, iptr]
+    L.orc_heuristic_on.restype = C.c_double
+    L.orc_ws_set_trace.argtypes = [C.c_void_p, dptr, C.c_int]
+    L.orc_ws_trace_n.argtypes = [C.c_void_p]
+    L.orc_ws_trace_n.restype = C.c_int
+    L.orc_chol_upper.argtypes = [dptr, C.c_int]
+    L.orc_chol_upper.restype = C.c_int
+    L.orc_update_triangles_on.argtypes = [dptr, C.c_int, C.POINTER(Params), C.POINTER(Tri), iptr, C.c_int,
+                                          dptr, C.POINTER(Tri), iptr]
+    L.orc_update_triangles_on.restype = C.c_double
+    return L
 
 
 def _d(a):
@@ -272,7 +305,40 @@ def sdpbound(SP_L, xfixed, solX, LB, diff, is_root, seed, params, ws=None):
     return out, frac, xb
 
 
-def sdpbound_trace(SP_L, xfixed, solX, LB, diff, is_root, seed, params, cap=4096):
+def lambda_eta(zeta, G, gamma, t):
+    """lambda_eta (bundle.c:292-373) for G (m x k, column-major), returns
+    (dgamma, lambda, eta, t_out, status)."""
+    G = f64(G)
+    m, k = G.shape
+    zeta = np.ascontiguousarray(zeta, dtype=np.float64)
+    gamma = np.ascontiguousarray(gamma, dtype=np.float64)
+    dg, eta, lam = np.zeros(max(m, 1)), np.zeros(max(m, 1)), np.zeros(k)
+    tt = C.c_double(t)
+    st = lib().orc_lambda_eta(m, k, _d(zeta), _d(G), m, _d(gamma), _d(dg), _d(lam), _d(eta), C.byref(tt))
+    return dg[:m], lam, eta[:m], tt.value, st
+
+
+def update_triangles_on(X, params, cuts, y, max_tri):
+    """updateTriangleInequalities (cutting_planes.c:132-214) on X and a cut
+    list of (i, j, k, type) with multipliers y.  Returns (cuts_out as
+    (i, j, k, type, y) tuples, list as (i, j, k, type, value), max violation)."""
+    X = f64(X)
+    n = X.shape[0]
+    ca = (Tri * max(max_tri, 1))()
+    for c, (i, j, k, t) in enumerate(cuts):
+        ca[c].i, ca[c].j, ca[c].k, ca[c].type = i, j, k, t
+    nc = C.c_int(len(cuts))
+    la = (Tri * max(params.TriIneq, 1))()
+    ls = C.c_int()
+    y = np.ascontiguousarray(np.asarray(y, dtype=np.float64).reshape(-1) if len(cuts) else np.zeros(1))
+    mx = lib().orc_update_triangles_on(_d(X), n, C.byref(params), ca, C.byref(nc), max_tri, _d(y), la,
+                                       C.byref(ls))
+    out = [(ca[c].i, ca[c].j, ca[c].k, ca[c].type, ca[c].y) for c in range(nc.value)]
+    lst = [(la[c].i, la[c].j, la[c].k, la[c].type, la[c].value) for c in range(ls.value)]
+    return out, lst, mx
+
+
+def sdpbound_trace(SP_L, xfixed, solX, LB, diff, is_root, seed, params, cap=40000):
     """sdpbound with the step trace: returns (out, frac, xbest, rows[r, 8])."""
     SP_L = f64(SP_L)
     ws = lib().orc_ws_create(SP_L.shape[0], 400, 50000, 50000, 50000, C.byref(params))

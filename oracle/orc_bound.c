@@ -179,7 +179,7 @@ static double update_triangles(orc_ws *w, const double *y)
     int subtracted = 0, next = 0;
     for (int c = 0; c < w->NIneq; ++c) {
         w->Cuts[c].y = y[c];
-        if (w->Cuts[c].y < 1e-5) ++subtracted;
+        if (w->Cuts[c].y < ORC_CUT_PURGE) ++subtracted;
         else {
             orc_tri keep = w->Cuts[c];
             w->Cuts[next].type = keep.type; w->Cuts[next].i = keep.i;
@@ -209,6 +209,30 @@ static double update_triangles(orc_ws *w, const double *y)
     return maxall;
 }
 
+/* Test helper: updateTriangleInequalities (cutting_planes.c:132-214) on a
+ * given n x n matrix X and cut list.  cuts (capacity max_tri) holds *ncuts
+ * cuts with multipliers y[*ncuts]; it is purged and extended in place.  list
+ * (capacity p->TriIneq) receives the separated list and *list_size its
+ * length.  Returns the largest violation. */
+double orc_update_triangles_on(const double *X, int n, const orc_params *p, orc_tri *cuts, int *ncuts,
+                               int max_tri, const double *y, orc_tri *list, int *list_size)
+{
+    orc_ws *w = orc_ws_create(n, 1, max_tri, 1, 1, p);
+    if (!w) return -ORC_BIG_NUMBER;
+    w->n = n;
+    memcpy(w->X, X, sizeof(double) * n * n);
+    memcpy(w->Cuts, cuts, sizeof(orc_tri) * *ncuts);
+    w->NIneq = *ncuts;
+    const double mx = update_triangles(w, y);
+    memcpy(cuts, w->Cuts, sizeof(orc_tri) * w->NIneq);
+    *ncuts = w->NIneq;
+    int size = 0;
+    orc_get_violated_triangles(X, n, p->violated_TriIneq, p->TriIneq, list, &size);
+    *list_size = size;
+    orc_ws_destroy(w);
+    return mx;
+}
+
 /* cutting_planes.c:316-406 (K=5) and 509-602 (K=7) */
 #define DEFINE_KGON_UPDATE(NAME, TYPE_T, K, CUTS, LIST, COUNT, MAXC, TRIALS, LISTCAP, SEP)   \
 static double NAME(orc_ws *w, const double *y, int offset, uint64_t key)                    \
@@ -216,7 +240,7 @@ static double NAME(orc_ws *w, const double *y, int offset, uint64_t key)        
     int subtracted = 0, next = 0;                                                           \
     for (int c = 0; c < w->COUNT; ++c) {                                                    \
         w->CUTS[c].y = y[offset + c];                                                       \
-        if (w->CUTS[c].y < 1e-5) ++subtracted;                                              \
+        if (w->CUTS[c].y < ORC_CUT_PURGE) ++subtracted;                                     \
         else {                                                                              \
             TYPE_T keep = w->CUTS[c];                                                       \
             w->CUTS[next].type = keep.type; w->CUTS[next].y = keep.y;                       \
@@ -390,7 +414,7 @@ void orc_sdpbound(orc_ws *w, const double *SP_L, int n0, const orc_node_in *in,
             if (w->bundle == w->max_bundle) { w->status = ORC_ERR_BUNDLE_FULL; goto END; }
             ++w->bundle;
             if (w->bundle > w->max_bundle_seen) w->max_bundle_seen = w->bundle;
-            t *= 1.05;
+            t *= ORC_T_REBUILD;
         }
         bdl_iter += count % 2;
         bdl_iter = (bdl_iter < p->max_bundle_iter) ? bdl_iter : p->max_bundle_iter;

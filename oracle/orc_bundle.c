@@ -200,8 +200,8 @@ int orc_lambda_eta(int m, int k, const double *zeta, const double *G, int ldG,
             nrm += dgamma[i] * dgamma[i];
         }
         dir_curr = sqrt(nrm);
-        if (ABSV(dir_curr - dir_prev) / (1.0 + dir_curr) < 1e-5) done = 1;
-        if (cnt >= 50) { done = 1; *t *= 0.95; }
+        if (ABSV(dir_curr - dir_prev) / (1.0 + dir_curr) < ORC_LE_TOL) done = 1;
+        if (cnt >= ORC_LE_MAXIT) { done = 1; *t *= ORC_T_NOCONV; }
         dir_prev = dir_curr;
     }
     free(Q); free(tmp); free(c);
@@ -293,7 +293,9 @@ int orc_bundle_method(orc_ws *w, double *t, int bdl_iter)
         double lmax = -ORC_BIG_NUMBER;
         for (int i = 0; i < k; ++i) lmax = (w->lambda[i] > lmax) ? w->lambda[i] : lmax;
 
-        const int serious = (w->f - f_test > 0.05 * del) ? 1 : 0;
+        const int serious = (w->f - f_test > ORC_SERIOUS_FRAC * del) ? 1 : 0;
+        orc_trace(w, 5, w->count, del, w->f, f_test, lmax, k, serious);
+        for (int i = 0; i < k; ++i) orc_trace(w, 6, w->count, i, w->lambda[i], lmax, k, serious, 0);
         if (serious) {                          /* serious step */
             memcpy(w->gamma, w->gamma_test, sizeof(double) * m);
             w->f = f_test;
@@ -302,10 +304,10 @@ int orc_bundle_method(orc_ws *w, double *t, int bdl_iter)
                 for (int j = 0; j < k; ++j) s += w->X_bundle[i + (size_t)j * nn] * w->lambda[j];
                 w->X[i] = s;
             }
-            *t *= 1.01;
+            *t *= ORC_T_STEP;
             int subtracted = 0, next = 0;
             for (int i = 0; i < k; ++i) {
-                if (w->lambda[i] < 0.01 * lmax) ++subtracted;
+                if (w->lambda[i] < ORC_PURGE_LAMBDA * lmax) ++subtracted;
                 else {
                     if (i > next) bundle_copy(w, m, nn, i, next);
                     ++next;
@@ -319,10 +321,10 @@ int orc_bundle_method(orc_ws *w, double *t, int bdl_iter)
             w->F[w->bundle] = lx_dot(w->L, w->X_test, n);
             ++w->bundle;
         } else {                                /* null step */
-            *t /= 1.01;
+            *t /= ORC_T_STEP;
             int subtracted = 0, next = 0;
             for (int i = 0; i < k - 1; ++i) {
-                if (w->lambda[i] < 0.01 * lmax) ++subtracted;
+                if (w->lambda[i] < ORC_PURGE_LAMBDA * lmax) ++subtracted;
                 else {
                     if (i > next) bundle_copy(w, m, nn, i, next);
                     ++next;

@@ -6,6 +6,35 @@
 
 #define ORC_BIG_NUMBER 1e+9
 
+/* Constants of the reference's algorithm.  They can be overridden at build
+ * time only so that the tests can show that their checks notice a changed
+ * constant (tests/test_oracle_pins.py builds such variants); the oracle
+ * itself is always built with these values. */
+#ifndef ORC_SERIOUS_FRAC
+#define ORC_SERIOUS_FRAC 0.05   /* serious step test, bundle.c:93 */
+#endif
+#ifndef ORC_T_STEP
+#define ORC_T_STEP 1.01         /* t *= 1.01 (serious) / t /= 1.01 (null), bundle.c:112, 172 */
+#endif
+#ifndef ORC_PURGE_LAMBDA
+#define ORC_PURGE_LAMBDA 0.01   /* bundle purge lambda < 0.01 max(lambda), bundle.c:118, 177 */
+#endif
+#ifndef ORC_T_NOCONV
+#define ORC_T_NOCONV 0.95       /* lambda-eta loop without convergence, bundle.c:362-365 */
+#endif
+#ifndef ORC_LE_MAXIT
+#define ORC_LE_MAXIT 50         /* bundle.c:362 */
+#endif
+#ifndef ORC_LE_TOL
+#define ORC_LE_TOL 1e-5         /* bundle.c:358 */
+#endif
+#ifndef ORC_CUT_PURGE
+#define ORC_CUT_PURGE 1e-5      /* purge of inactive cuts, cutting_planes.c:151, 336, 529 */
+#endif
+#ifndef ORC_T_REBUILD
+#define ORC_T_REBUILD 1.05      /* t after a cut change, bounding.c:318 */
+#endif
+
 struct orc_ws {
     orc_params p;
     int n0;                    /* vertices of the original problem (SP->n) */

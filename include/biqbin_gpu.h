@@ -75,6 +75,9 @@ typedef struct BiqBinParameters {
 #define BQB_FLAG_TRI_TRUNC 2
 #define BQB_FLAG_PENT_TRUNC 4
 #define BQB_FLAG_HEPT_TRUNC 8
+/* An interior-point solve of the node with products on the bf16 matrix
+ * cores failed and was repeated with FP64 products. */
+#define BQB_FLAG_FP64_RETRY 16
 /* ints per node in the stats arrays of the batch API */
 #define BQB_NSTATS 9
 
@@ -174,6 +177,11 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
  * branched. */
 int bqb_solve_cap(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
                   int max_bundle, int *sol, bqb_solve_stats *stats);
+/* The same with the precision of the bounds (bqb_ctx_set_precision):
+ * slices 2..4 bound on the bf16 matrix cores with certified bounds,
+ * certify = 1 certifies FP64 bounds. */
+int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
+                 int max_bundle, int slices, int certify, int *sol, bqb_solve_stats *stats);
 
 /* ---- batch: interior-point solve of the basic SDP ---------------------- */
 /* Solves max <L_b, X_b> s.t. diag(X_b) = e, X_b psd for `nodes` independent
@@ -188,6 +196,21 @@ int bqb_ipm_ld_for(int nmax);
 
 int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
                   int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap);
+
+/* Mixed-precision form of bqb_ipm_batch (no counterpart in the reference;
+ * follows src/ipm_mc_pk.c:28-359 with its two n x n products,
+ * ipm_mc_pk.c:186-189 and 253-257, on the bf16 matrix cores): every FP64
+ * operand is split into `slices` (2..4) bf16 slices whose products the
+ * matrix cores sum exactly in FP32; the slice products are combined in FP64
+ * (error <= (4s+1) n 2^(-8s) times the operands' row/column maxima).  The
+ * inversions, the Cholesky line searches and everything else stay FP64.
+ * slices = 0 runs the FP64 products.  phi_cert[b] (may be NULL) receives a
+ * certified upper bound of the basic SDP value: e'y plus n times a rigorous
+ * bound on -lambda_min(Diag(y) - L) from an FP64 Cholesky factor and a
+ * residual enclosed with directed rounding.  *ms (may be NULL): device time
+ * of the IPM launch. */
+int bqb_ipm_batch_mixed(int nodes, const int *n, const double *L, double *X, double *phi, double *phi_cert,
+                        int *iters, int *trials, int *status, double *y_out, int slices, double *ms);
 
 /* No counterpart in the reference.  The batched IPM of bqb_ipm_batch for
  * callers that cannot allocate device memory (code under HIP graph capture,
@@ -304,7 +327,8 @@ void bqb_ctx_trajectory_clear(bqb_ctx *c);
  * 103 triangle update, 104/105 pentagonal/heptagonal update, 106 op_Bt
  * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP
  * (arg: an upper bound on every bundle size, 0 if unknown; a bound <= 16
- * selects the small-LDS QP kernel), 109 bundle step), run the whole bound,
+ * selects the small-LDS QP kernel), 109 bundle step, 110/111 certificate of
+ * the basic bound / of a new bundle center), run the whole bound,
  * read results, and copy one node's
  * device buffer (ids documented in bqb_sdpbound.c) to or from the host. */
 int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
@@ -316,6 +340,19 @@ int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, doub
 int bqb_ctx_download(bqb_ctx *c, int id, int node, void *host, size_t bytes);
 int bqb_ctx_upload(bqb_ctx *c, int id, int node, const void *host, size_t bytes);
 int bqb_ctx_sync(bqb_ctx *c);
+/* Precision of a context's bounds (no counterpart in the reference; call
+ * while no node is in flight): slices = 0 runs the IPM products in FP64 (the
+ * reference's arithmetic), 2..4 on the bf16 matrix cores as in
+ * bqb_ipm_batch_mixed.  certify = 1 (implied by slices > 0) makes every
+ * value the node prunes on and returns a certified upper bound: the basic
+ * bound and every new bundle center (a serious step, bundle.c:93-135) get a
+ * certificate  sum(gamma) + e'y + n * bound(-lambda_min(Diag(y) - L + B'gamma))
+ * computed with directed rounding (src/bounding.c:109, 180, 197 prune on it). */
+int bqb_ctx_set_precision(bqb_ctx *c, int slices, int certify);
+/* bf16 matrix-core flops counted since the profile started (2 n^3 per slice
+ * pair and product). */
+int bqb_ctx_ipm_flops_mx(bqb_ctx *c, double *flops_mx);
+
 /* IPM launch profile (no counterpart in the reference; for benchmarks):
  * start with max_launches > 0 (event pair per IPM launch, device flop
  * counter reset), stop with 0; read launches, summed ms and algorithmic
@@ -347,6 +384,11 @@ int bqb_ipm_debug(int n, const double *L, int max_iters, double *X, double *Zi, 
  * (copy_probe_kernel); *ms = average time per copy.  Calibrates HBM byte
  * counters: each copy moves exactly 2 x bytes. */
 int bqb_copy_probe(size_t bytes, int reps, double *ms);
+
+/* Diagnostic (no counterpart in the reference): C = A B for n x n
+ * column-major matrices (n <= 256) through the sliced bf16 matrix-core
+ * product of the mixed-precision IPM with `slices` (1..4) slices. */
+int bqb_mx_selftest(int n, int slices, const double *A, const double *B, double *C);
 
 /* Diagnostic: apply one workgroup-distributed primitive to an n x n matrix
  * (column-major) with register tile `tile` (1604, 1607 or 3204 = P*100 + R):

@@ -20,6 +20,10 @@ def declare(L):
     L.bqb_ipm_ld_for.argtypes = [C.c_int]
     L.bqb_ipm_ld_for.restype = C.c_int
     vp = C.c_void_p
+    L.bqb_ipm_batch_mixed.argtypes = [C.c_int, N.iptr, N.dptr, N.dptr, N.dptr, N.dptr, N.iptr, N.iptr, N.iptr,
+                                      N.dptr, C.c_int, N.dptr]
+    L.bqb_ipm_batch_mixed.restype = C.c_int
+    L.bqb_mx_selftest.argtypes = [C.c_int, C.c_int, N.dptr, N.dptr, N.dptr]
     L.bqb_ipm_noalloc_bytes.argtypes = [C.c_int, C.c_int]
     L.bqb_ipm_noalloc_bytes.restype = C.c_size_t
     L.bqb_ipm_batch_noalloc.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
@@ -69,6 +73,39 @@ def ipm_batch(Ls, trace_cap=0):
         oy += n
     traces = [tr[b * 2 * trace_cap: b * 2 * trace_cap + 2 * iters[b]] for b in range(nodes)] if trace_cap else None
     return dict(X=Xs, phi=phi, iters=iters, trials=trials, status=status, y=ys, trace=traces)
+
+
+def mx_product(A, B, slices=3):
+    """C = A B through the sliced bf16 matrix-core product (diagnostic)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    n = A.shape[0]
+    Cm = np.zeros((n, n), order="F")
+    N.check(N.lib().bqb_mx_selftest(n, slices, _d(A), _d(B), _d(Cm)))
+    return Cm
+
+
+def ipm_batch_mixed(Ls, slices=4):
+    """Batched IPM with the products on the bf16 matrix cores (slices > 0)
+    or in FP64 (slices = 0), and certified bounds phi_cert.  Returns dict
+    with X (list), phi, phi_cert, iters, trials, status, y (list), ms."""
+    ns = np.array([m.shape[0] for m in Ls], dtype=np.int32)
+    flat = np.concatenate([np.asfortranarray(m, dtype=np.float64).ravel(order="F") for m in Ls])
+    nodes = len(Ls)
+    X = np.zeros(flat.size)
+    phi, cert = np.zeros(nodes), np.zeros(nodes)
+    iters, trials, status = (np.zeros(nodes, dtype=np.int32) for _ in range(3))
+    y = np.zeros(int(ns.sum()))
+    ms = C.c_double()
+    N.check(N.lib().bqb_ipm_batch_mixed(nodes, _i(ns), _d(flat), _d(X), _d(phi), _d(cert), _i(iters), _i(trials),
+                                        _i(status), _d(y), slices, C.byref(ms)))
+    Xs, ys, o, oy = [], [], 0, 0
+    for n in ns:
+        Xs.append(X[o:o + n * n].reshape((n, n), order="F"))
+        ys.append(y[oy:oy + n])
+        o += n * n
+        oy += n
+    return dict(X=Xs, phi=phi, phi_cert=cert, iters=iters, trials=trials, status=status, y=ys, ms=ms.value)
 
 
 def ipm_debug(L, max_iters):
@@ -146,14 +183,14 @@ class NodeState(C.Structure):
         "improved", "ipm_calls", "ipm_iters", "max_cuts", "max_bundle_seen", "is_root", "serious",
         "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags")] + [(f, C.c_double) for f in (
         "fixedvalue", "f", "f_test", "t", "bound", "basic_bound", "oldf", "LB", "diff", "viol3",
-        "viol5", "gap", "del_", "lmax")] + [("seed", C.c_uint64)]
+        "viol5", "gap", "del_", "lmax", "f_cert")] + [("seed", C.c_uint64)]
 
 
 BUF = dict(state=0, L=1, L0=2, X=3, Xt=4, tri=5, tri_y=6, pent=7, pent_y=8, hept=9, hept_y=10,
            gamma=11, gtest=12, g=13, G=14, F=15, lam=16, tri_list=17, tri_lval=18, kg_list=19,
            kg_lval=20, fracsol=21, xbest=22, xheur=23, mask=24, phi=25, Xb=26, nvec=27, dgamma=28,
            sa_val=29)
-PH = dict(SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OUTER_A=5, OUTER_B=6,
+PH = dict(CERT_BASIC=110, CERT_CENTER=111, SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OUTER_A=5, OUTER_B=6,
           SEP_PENT=7, SEP_HEPT=8, OUTER_C=9, REBUILD=10, FINISH=11, FCT_FINISH=12, LOAD=13,
           BUNDLE_NEXT=14, IPM_BASIC=100, IPM_L0=101,
           HEUR=102, TRISEP=103, PENT=104, HEPT=105, CSR=106, OPBT=107, QP=108, STEP=109)
@@ -187,7 +224,7 @@ class SolveStats(C.Structure):
 
 NMAX = 1024
 NSTATS = 9   # ints per node in the stats arrays (BQB_NSTATS)
-FLAG_BUNDLE_FULL, FLAG_TRI_TRUNC, FLAG_PENT_TRUNC, FLAG_HEPT_TRUNC = 1, 2, 4, 8
+FLAG_BUNDLE_FULL, FLAG_TRI_TRUNC, FLAG_PENT_TRUNC, FLAG_HEPT_TRUNC, FLAG_FP64_RETRY = 1, 2, 4, 8, 16
 
 
 class Problem(C.Structure):
@@ -215,6 +252,8 @@ def _declare_host(L):
                             C.POINTER(SolveStats)]
     L.bqb_solve_cap.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, C.c_int,
                                 N.iptr, C.POINTER(SolveStats)]
+    L.bqb_solve_ex.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, C.c_int,
+                               C.c_int, C.c_int, N.iptr, C.POINTER(SolveStats)]
     for f in ("Evaluate", "SDPbound"):
         getattr(L, f).argtypes = [C.POINTER(BabNode), C.POINTER(Problem), C.POINTER(Problem), C.c_int]
         getattr(L, f).restype = C.c_double
@@ -272,7 +311,7 @@ def evaluate_solution(SP_L, sol):
     return N.lib().bqb_evaluate_solution(SP_L.shape[0], _d(SP_L), _i(s))
 
 
-def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0):
+def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certify=0):
     """Branch and bound on the GPU (bqb_solve / bqb_solve_cap).  Returns
     (stats, solution) with the solution over the first n0-1 vertices (the
     last is on side 0)."""
@@ -281,7 +320,8 @@ def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0):
     p = params if params is not None else make_params()
     sol = np.zeros(n0 - 1, dtype=np.int32)
     st = SolveStats()
-    N.check(N.lib().bqb_solve_cap(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, _i(sol), C.byref(st)))
+    N.check(N.lib().bqb_solve_ex(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, slices, certify, _i(sol),
+                                 C.byref(st)))
     return st, sol
 
 
@@ -323,6 +363,8 @@ def _declare_ctx(L):
     L.bqb_ctx_sync.argtypes = [vp]
     L.bqb_ctx_profile_ipm.argtypes = [vp, C.c_int]
     L.bqb_ctx_ipm_stats.argtypes = [vp, N.iptr, N.dptr, N.dptr]
+    L.bqb_ctx_set_precision.argtypes = [vp, C.c_int, C.c_int]
+    L.bqb_ctx_ipm_flops_mx.argtypes = [vp, N.dptr]
 
 
 class BoundContext:
@@ -492,6 +534,21 @@ class BoundContext:
             tot[1] += ms.value
             tot[2] += fl.value
         return tuple(tot)
+
+    def ipm_flops_mx(self):
+        """bf16 matrix-core flops since profiling started, over all groups."""
+        tot = 0.0
+        for h in self.handles:
+            f = C.c_double()
+            N.check(N.lib().bqb_ctx_ipm_flops_mx(h, C.byref(f)))
+            tot += f.value
+        return tot
+
+    def set_precision(self, slices=0, certify=0):
+        """IPM products in FP64 (slices 0) or on the bf16 matrix cores
+        (2..4 slices), and certified bounds (implied by slices > 0)."""
+        for h in self.handles:
+            N.check(N.lib().bqb_ctx_set_precision(h, int(slices), int(certify)))
 
     def matrix(self, buf, node, n):
         a = self.get(buf, node, np.float64, self.ld * self.ld).reshape((self.ld, self.ld), order="F")

@@ -46,6 +46,7 @@ typedef struct NodeState {
     int flags;          /* BQB_FLAG_* events of the node (reported in stats[8]) */
     double fixedvalue, f, f_test, t, bound, basic_bound, oldf;
     double LB, diff, viol3, viol5, gap, del, lmax;
+    double f_cert;      /* certified upper bound of f (when certifying) */
     uint64_t seed;
 } NodeState;
 
@@ -70,6 +71,9 @@ typedef struct BatchDims {
     int npairs;         /* ld*(ld-1)/2 */
     int sa_chains5, sa_chains7;   /* SA chains per pentagonal / heptagonal round */
     int tmp_ld;         /* per-node length of tmpm */
+    int mix_slices;     /* IPM products: 0 FP64, 2..4 bf16 slices on the matrix cores */
+    int certify;        /* prune on certified bounds (f_cert) instead of f */
+    int work_slabs;     /* ld x ld slabs of IPM work per node */
 } BatchDims;
 
 /* Device pointers of a batch (all arrays are per node at the stated stride). */
@@ -80,9 +84,11 @@ typedef struct BatchPtrs {
     const int *xfixed, *solX;       /* [nodes*(n0-1)] */
     int *nvec;                      /* [nodes] n of each node (for the IPM kernel) */
     int *mask;                      /* [nodes] phase mask */
+    int *mask2;                     /* [nodes] nodes re-solved with FP64 products */
     double *L, *L0, *X, *Xt, *Zh;   /* [nodes*ld*ld] each */
     double *ipm_work;               /* [nodes*3*ld*ld] */
     double *phi;                    /* [nodes] */
+    double *ytest;                  /* [nodes*ld] dual vector y of the last IPM */
     int *ipm_iters, *ipm_trials, *ipm_status;   /* [nodes] */
     double *Xb;                     /* [nodes*max_bundle*ld*ld] bundle matrices */
     double *G;                      /* [nodes*max_bundle*max_m], column i at i*m (ld = m) */

@@ -299,7 +299,7 @@ __global__ __launch_bounds__(BT, (KM == KMAX) ? 2 : 3) void bundle_qp_kernel(Bat
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (split && (KM == KMAX ? k <= 16 : k > KM)) return;   // the other launch's node
     if (k > KM || k < 1) {
-        if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node);
+        if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node, d.certify);
         return;
     }
     const double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(BT, (KM == KMAX) ? 2 : 3) void bundle_qp_kernel(Bat
     double *tmp = p.tmpm + (size_t)node * d.tmp_ld;
     double t = s.t;
     if (k > PE && (size_t)(k - PE) * BT > (size_t)d.tmp_ld) {
-        if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node);
+        if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node, d.certify);
         return;
     }
     for (int j = PE; j < k; ++j) tmp[(size_t)(j - PE) * BT + threadIdx.x] = 0.0;
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
     }
     int bundle = k - subtracted;
     if (bundle >= d.max_bundle) {
-        if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node);
+        if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node, d.certify);
         return;
     }
     int pos;

@@ -181,6 +181,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             sp->ipm_calls = sp->ipm_iters = 0;
             sp->max_cuts = sp->max_bundle_seen = 0;
             sp->viol3 = sp->viol5 = 0.0;
+            sp->f_cert = INFINITY;       // until a certificate is computed
             sp->cuts_changed = 1;
             p.nvec[node] = n;
             p.mask[node] = 1;
@@ -245,8 +246,9 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         if (threadIdx.x == 0) {
             int m = 0;
             if (s.active && !s.status) {
-                const double bound = s.f + s.fixedvalue;
+                const double bound = node_bound(s, d.certify);
                 sp->bound = bound;
+                if (d.certify) sp->basic_bound = bound;
                 if (bound < s.LB + 1.0) { sp->prune = 1; sp->active = 0; }
                 else if (P.use_diff && !s.is_root && bound > s.LB + s.diff + 1.0) { sp->giveup = 1; sp->active = 0; }
                 else m = 1;
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         if (threadIdx.x == 0) {
             int h = 0;
             if (s.active) {
-                const double bound = s.f + s.fixedvalue;
+                const double bound = node_bound(s, d.certify);
                 sp->bound = bound;
                 sp->prune = (bound < s.LB + 1.0) ? 1 : 0;
                 sp->heur_idx = s.count;
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             if (threadIdx.x == 0) {
                 sp->done = 1;
                 sp->active = 0;
-                sp->bound = s.f + s.fixedvalue;
+                sp->bound = node_bound(s, d.certify);
                 if (s.is_root && !s.status) sp->diff = s.basic_bound - sp->bound;
                 p.mask[node] = 0;
             }
@@ -406,7 +408,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         if (!s.active) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
         const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
         if (s.bundle >= d.max_bundle) {
-            if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node);
+            if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node, d.certify);
             return;
         }
         double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
@@ -460,7 +462,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
 
     if (phase == PH_FINISH) {
         if (threadIdx.x == 0) {
-            if (s.active && s.stage == BQB_STAGE_LOOP) { sp->active = 0; sp->bound = s.f + s.fixedvalue; }
+            if (s.active && s.stage == BQB_STAGE_LOOP) { sp->active = 0; sp->bound = node_bound(s, d.certify); }
             p.mask[node] = 0;
         }
         return;

@@ -10,18 +10,26 @@
 // the terms written out in operators.c -- so sums round identically.
 #pragma once
 #include "bqb_batch.h"
+#include "bqb_round.h"
 
 namespace bqb {
+
+// The node's upper bound f + fixedvalue (bounding.c:177, 339); when the
+// context certifies, the certified f (rounded up) takes the place of f.
+__device__ __forceinline__ double node_bound(const NodeState &s, int certify)
+{
+    return certify ? bqb_add_ru(s.f_cert, s.fixedvalue) : s.f + s.fixedvalue;
+}
 
 // The node's bundle is full: it leaves SDPbound as a give-up with the bound
 // of its current stability center, f + fixedvalue (a valid bound for any
 // multipliers gamma >= 0), and the event is flagged.  One thread calls it.
-__device__ __forceinline__ void bundle_full_giveup(NodeState *sp, const NodeState &s, int *mask)
+__device__ __forceinline__ void bundle_full_giveup(NodeState *sp, const NodeState &s, int *mask, int certify)
 {
     sp->giveup = 1;
     sp->done = 1;
     sp->active = 0;
-    sp->bound = s.f + s.fixedvalue;
+    sp->bound = node_bound(s, certify);
     sp->flags = s.flags | BQB_FLAG_BUNDLE_FULL;
     *mask = 0;
 }

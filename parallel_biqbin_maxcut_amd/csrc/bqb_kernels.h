@@ -29,7 +29,11 @@ enum {
 
 /* Batched IPM: node b has size n[b] <= ld; its L and X are column-major
  * ld x ld slabs at offset b*ld*ld (doubles), zero outside the leading n x n
- * block (the kernel keeps X zero there).  work holds 3*ld*ld doubles per node. */
+ * block (the kernel keeps X zero there).  work holds BQB_IPM_WORK_SLABS(s)
+ * ld x ld slabs per node: inv(Z), inv(Z o X), dX, and with mixed products
+ * (mix_slices = s > 0) three more for the bf16 slices of the operands. */
+#define BQB_IPM_WORK_SLABS(s) ((s) > 0 ? 6 : 3)
+#define BQB_MIX_MAX_SLICES 4
 typedef struct IpmBatchArgs {
     const int *n;
     const double *L;
@@ -47,7 +51,9 @@ typedef struct IpmBatchArgs {
     int trace_cap;
     int max_iters;         /* 0: default cap; k > 0: stop after k iterations */
     double *dbg;           /* optional [nodes * 4 * ld]: initial y, zd, diag(L), and phi/psi/mu */
-    unsigned long long *flops;  /* optional: += algorithmic flops of each node's solve */
+    unsigned long long *flops;  /* optional: += algorithmic FP64 flops of each node's solve */
+    int mix_slices;        /* 0: FP64 products; 2..4: products on bf16 matrix cores with this many slices */
+    unsigned long long *flops_mx;   /* optional (mixed): += bf16 matrix-core flops, 2 n^3 per slice pair */
 } IpmBatchArgs;
 
 /* Algorithmic FP64 flops of one IPM solve of size n: per iteration two SPD
@@ -55,6 +61,20 @@ typedef struct IpmBatchArgs {
  * Cholesky test of the line searches. */
 #define BQB_IPM_FLOPS(n, iters, trials) \
     ((double)(n) * (n) * (n) * (6.0 * (iters) + (trials) / 3.0))
+/* FP64 part with the products on the matrix cores: the two inversions. */
+#define BQB_IPM_FLOPS_MIXED(n, iters, trials) \
+    ((double)(n) * (n) * (n) * (2.0 * (iters) + (trials) / 3.0))
+/* bf16 matrix-core flops of one mixed solve: two products of s(s+1)/2
+ * slice pairs, 2 n^3 each, per iteration. */
+#define BQB_IPM_FLOPS_MX(n, iters, s) ((double)(n) * (n) * (n) * 2.0 * (s) * ((s) + 1) * (iters))
+
+/* After a mixed-precision launch: solve again, with FP64 products, the
+ * nodes whose mixed solve failed (mask2: [nodes] scratch; flags: optional
+ * per-node flag words at the given int stride, BQB_FLAG_FP64_RETRY set). */
+hipError_t bqb_launch_ipm_retry(const IpmBatchArgs *a, int nodes, int tile, int *mask2, int *flags, int stride,
+                                hipStream_t s);
+hipError_t bqb_launch_cert_ipm(const int *nvec, const double *L, const double *y, long ystride, double *work,
+                               int ld, int wslabs, const int *status, double *out, int nodes, hipStream_t s);
 
 int bqb_ipm_tile_for(int nmax);
 hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int tile, hipStream_t s);

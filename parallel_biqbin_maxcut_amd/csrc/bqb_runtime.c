@@ -104,12 +104,25 @@ int bqb_ipm_ld_for(int nmax)
 
 static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, double *phi,
                           int *iters, int *trials, int *status, double *y_out, double *alpha_trace,
-                          int trace_cap, int max_iters, double *work_out);
+                          int trace_cap, int max_iters, double *work_out, int mix_slices, double *phi_cert,
+                          double *ms_out);
 
 int bqb_ipm_batch(int nodes, const int *n, const double *L, double *X, double *phi, int *iters,
                   int *trials, int *status, double *y_out, double *alpha_trace, int trace_cap)
 {
-    return ipm_batch_impl(nodes, n, L, X, phi, iters, trials, status, y_out, alpha_trace, trace_cap, 0, NULL);
+    return ipm_batch_impl(nodes, n, L, X, phi, iters, trials, status, y_out, alpha_trace, trace_cap, 0, NULL, 0,
+                          NULL, NULL);
+}
+
+int bqb_ipm_batch_mixed(int nodes, const int *n, const double *L, double *X, double *phi, double *phi_cert,
+                        int *iters, int *trials, int *status, double *y_out, int slices, double *ms)
+{
+    if (slices < 0 || slices == 1 || slices > BQB_MIX_MAX_SLICES) {
+        bqb_set_error("slices = %d: 0 (FP64 products) or 2..%d", slices, BQB_MIX_MAX_SLICES);
+        return -1;
+    }
+    return ipm_batch_impl(nodes, n, L, X, phi, iters, trials, status, y_out, NULL, 0, 0, NULL, slices, phi_cert,
+                          ms);
 }
 
 int bqb_ipm_debug(int n, const double *L, int max_iters, double *X, double *Zi, double *Mi,
@@ -120,7 +133,8 @@ int bqb_ipm_debug(int n, const double *L, int max_iters, double *X, double *Zi, 
     double *w = malloc(sizeof(double) * (3 * (size_t)ld * ld + 4 * (size_t)ld));
     if (!w) { bqb_set_error("malloc"); return -1; }
     int st = 0;
-    int rc = ipm_batch_impl(1, &n, L, X, phi, iters, NULL, &st, y, alpha_trace, max_iters, max_iters, w);
+    int rc = ipm_batch_impl(1, &n, L, X, phi, iters, NULL, &st, y, alpha_trace, max_iters, max_iters, w, 0, NULL,
+                            NULL);
     if (rc == 0) {
         double *outs[3] = {Zi, Mi, dX};
         for (int m = 0; m < 3; ++m)
@@ -135,8 +149,10 @@ int bqb_ipm_debug(int n, const double *L, int max_iters, double *X, double *Zi, 
 
 static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, double *phi,
                           int *iters, int *trials, int *status, double *y_out, double *alpha_trace,
-                          int trace_cap, int max_iters, double *work_out)
+                          int trace_cap, int max_iters, double *work_out, int mix_slices, double *phi_cert,
+                          double *ms_out)
 {
+    const int wslabs = BQB_IPM_WORK_SLABS(mix_slices);
     if (nodes <= 0) return 0;
     if (bqb_ensure_init()) return -1;
     int nmax = 0;
@@ -150,19 +166,22 @@ static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, d
     const size_t nn = (size_t)ld * ld;
     if (trace_cap < 0) trace_cap = 0;
 
-    /* device layout: n | L slabs | X slabs | work (3 slabs/node) | phi | iters | trials | status | y | trace */
+    /* device layout: n | L slabs | X slabs | work (wslabs/node) | phi | iters | trials | status | y | trace
+     * | debug | certified phi */
     size_t off_n = 0;
     size_t off_L = (off_n + sizeof(int) * nodes + 255) & ~(size_t)255;
     size_t off_X = off_L + sizeof(double) * nn * nodes;
     size_t off_W = off_X + sizeof(double) * nn * nodes;
-    size_t off_phi = off_W + sizeof(double) * 3 * nn * nodes;
+    size_t off_phi = off_W + sizeof(double) * wslabs * nn * nodes;
     size_t off_it = off_phi + sizeof(double) * nodes;
     size_t off_tr = off_it + sizeof(int) * nodes;
     size_t off_st = off_tr + sizeof(int) * nodes;
     size_t off_y = (off_st + sizeof(int) * nodes + 255) & ~(size_t)255;
     size_t off_a = off_y + sizeof(double) * ld * nodes;
     size_t off_dbg = off_a + sizeof(double) * 2 * (size_t)trace_cap * nodes;
-    size_t total = off_dbg + (work_out ? sizeof(double) * 4 * ld * nodes : 0);
+    size_t off_cert = off_dbg + (work_out ? sizeof(double) * 4 * ld * nodes : 0);
+    size_t off_m2 = off_cert + sizeof(double) * nodes;
+    size_t total = off_m2 + sizeof(int) * nodes;
     if (bqb_buf_reserve(&g_ipm_dev, total)) return -1;
     size_t hbytes = off_W;                      /* n + L slabs + X slabs */
     if (bqb_hbuf_reserve(&g_ipm_host, hbytes)) return -1;
@@ -199,8 +218,30 @@ static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, d
     a.trace_cap = trace_cap;
     a.max_iters = max_iters;
     a.dbg = work_out ? (double *)(d + off_dbg) : NULL;
+    a.mix_slices = mix_slices;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    if (ms_out && (bqb_check(hipEventCreate(&e0), "event") || bqb_check(hipEventCreate(&e1), "event"))) return -1;
+    if (ms_out) hipEventRecord(e0, 0);
     if (bqb_check(bqb_launch_ipm_batch(&a, nodes, tile, 0), "ipm_batch_kernel launch")) return -1;
+    /* mixed solves that failed are repeated with FP64 products */
+    if (mix_slices && bqb_check(bqb_launch_ipm_retry(&a, nodes, tile, (int *)(d + off_m2), NULL, 0, 0), "retry launch"))
+        return -1;
+    if (ms_out) hipEventRecord(e1, 0);
+    if (phi_cert &&
+        bqb_check(bqb_launch_cert_ipm(a.n, a.L, a.y_out, a.ystride, a.work, ld, wslabs, a.status,
+                                      (double *)(d + off_cert), nodes, 0), "certificate launch"))
+        return -1;
     if (bqb_check(hipDeviceSynchronize(), "ipm_batch_kernel")) return -1;
+    if (ms_out) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        *ms_out = ms;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    if (phi_cert &&
+        bqb_check(hipMemcpy(phi_cert, d + off_cert, sizeof(double) * nodes, hipMemcpyDeviceToHost), "D2H cert"))
+        return -1;
 
     double *hX = (double *)(h + off_L);        /* reuse the L staging area */
     if (bqb_check(hipMemcpy(hX, d + off_X, sizeof(double) * nn * nodes, hipMemcpyDeviceToHost), "hipMemcpy D2H")) return -1;
@@ -224,7 +265,7 @@ static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, d
         free(hy);
     }
     if (work_out &&
-        bqb_check(hipMemcpy(work_out, d + off_W, sizeof(double) * 3 * nn * nodes, hipMemcpyDeviceToHost), "D2H work"))
+        bqb_check(hipMemcpy(work_out, d + off_W, sizeof(double) * wslabs * nn * nodes, hipMemcpyDeviceToHost), "D2H work"))
         return -1;
     if (work_out &&
         bqb_check(hipMemcpy(work_out + 3 * nn * nodes, d + off_dbg, sizeof(double) * 4 * ld * nodes, hipMemcpyDeviceToHost), "D2H dbg"))
@@ -277,6 +318,34 @@ int bqb_dist_selftest(int tile, int op, int n, const double *in, double *out, in
 out:
     if (d) hipFree(d);
     free(h);
+    return rc;
+}
+
+hipError_t bqb_launch_mx_selftest(int n, int s, const double *A, const double *B, double *C, unsigned short *S);
+
+/* Diagnostic: C = A B through the sliced bf16 matrix-core product. */
+int bqb_mx_selftest(int n, int slices, const double *A, const double *B, double *C)
+{
+    if (bqb_ensure_init()) return -1;
+    if (n < 1 || n > 256 || slices < 1 || slices > BQB_MIX_MAX_SLICES) { bqb_set_error("mx selftest: n or slices out of range"); return -1; }
+    const size_t nn = sizeof(double) * (size_t)n * n;
+    const size_t sl = 2 * (size_t)2 * slices * 256 * 256;
+    const size_t soff = (3 * nn + 255) & ~(size_t)255;   /* the slices take 16-byte loads */
+    char *d = NULL;
+    int rc = -1;
+    if (bqb_check(hipMalloc((void **)&d, soff + sl), "hipMalloc")) return -1;
+    if (bqb_check(hipMemcpy(d, A, nn, hipMemcpyHostToDevice), "H2D") ||
+        bqb_check(hipMemcpy(d + nn, B, nn, hipMemcpyHostToDevice), "H2D") ||
+        bqb_check(hipMemset(d + 2 * nn, 0, nn), "memset"))
+        goto out;
+    if (bqb_check(bqb_launch_mx_selftest(n, slices, (const double *)d, (const double *)(d + nn), (double *)(d + 2 * nn),
+                                         (unsigned short *)(d + soff)), "mx selftest launch"))
+        goto out;
+    if (bqb_check(hipDeviceSynchronize(), "mx selftest")) goto out;
+    if (bqb_check(hipMemcpy(C, d + 2 * nn, nn, hipMemcpyDeviceToHost), "D2H")) goto out;
+    rc = 0;
+out:
+    hipFree(d);
     return rc;
 }
 

@@ -36,6 +36,7 @@ hipError_t bqb_launch_sa(const BatchDims *d, const BatchPtrs *p, int k, hipStrea
 hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, int spl_integer, hipStream_t s);
 hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, int kbound, hipStream_t s);
 hipError_t bqb_launch_bundle_step(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
+hipError_t bqb_launch_cert_node(const BatchDims *d, const BatchPtrs *p, int mode, hipStream_t s);
 
 struct bqb_ctx {
     BatchDims d;
@@ -56,7 +57,7 @@ struct bqb_ctx {
     int profile_ipm;
     hipEvent_t *pev;
     int pev_cap, pev_n;
-    unsigned long long *d_flops;
+    unsigned long long *d_flops;     /* [2]: FP64 and bf16 matrix-core flops */
     /* slot engine: occupied slots carry a caller tag (-1 = free); nodes
      * pushed since the last advance wait in the host staging arrays */
     long long *tag;            /* [max_nodes] */
@@ -184,6 +185,9 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
     d->npairs = d->ld * (d->ld - 1) / 2;
     d->sa_chains5 = 3 * c->params.Pent_Trials;
     d->sa_chains7 = 4 * c->params.Hepta_Trials;
+    d->mix_slices = 0;
+    d->certify = 0;
+    d->work_slabs = BQB_IPM_WORK_SLABS(0);
 
     const size_t N = (size_t)max_nodes, slab = (size_t)d->ld * d->ld, B = (size_t)(n0 - 1);
     struct { void **ptr; size_t bytes; } items[] = {
@@ -194,13 +198,16 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         {(void **)&c->p.solX, sizeof(int) * N * B},
         {(void **)&c->p.nvec, sizeof(int) * N},
         {(void **)&c->p.mask, sizeof(int) * N},
+        {(void **)&c->p.mask2, sizeof(int) * N},
         {(void **)&c->p.L, sizeof(double) * N * slab},
         {(void **)&c->p.L0, sizeof(double) * N * slab},
         {(void **)&c->p.X, sizeof(double) * N * slab},
         {(void **)&c->p.Xt, sizeof(double) * N * slab},
         {(void **)&c->p.Zh, sizeof(double) * N * slab},
-        {(void **)&c->p.ipm_work, sizeof(double) * N * 3 * slab},
+        /* room for the mixed-precision layout (bqb_ctx_set_precision) */
+        {(void **)&c->p.ipm_work, sizeof(double) * N * BQB_IPM_WORK_SLABS(1) * slab},
         {(void **)&c->p.phi, sizeof(double) * N},
+        {(void **)&c->p.ytest, sizeof(double) * N * d->ld},
         {(void **)&c->p.ipm_iters, sizeof(int) * N},
         {(void **)&c->p.ipm_trials, sizeof(int) * N},
         {(void **)&c->p.ipm_status, sizeof(int) * N},
@@ -235,7 +242,7 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         {(void **)&c->p.sa_val, sizeof(double) * N * BQB_SA_CHAINS},
         {(void **)&c->p.sa_perm, sizeof(int) * N * BQB_SA_CHAINS * 7},
         {(void **)&c->p.sa_status, sizeof(int) * N * BQB_SA_CHAINS},
-        {(void **)&c->d_flops, sizeof(unsigned long long)},
+        {(void **)&c->d_flops, 2 * sizeof(unsigned long long)},
         {(void **)&c->p.ld_st, sizeof(NodeState) * N},
         {(void **)&c->p.ld_xf, sizeof(int) * N * B},
         {(void **)&c->p.ld_sx, sizeof(int) * N * B},
@@ -321,12 +328,22 @@ static int ipm_launch(bqb_ctx *c, const double *L, double *X)
     a.trials = c->p.ipm_trials;
     a.status = c->p.ipm_status;
     a.active = c->p.mask;
+    a.mix_slices = c->d.mix_slices;
+    if (c->d.certify) {            /* the certificate needs y */
+        a.y_out = c->p.ytest;
+        a.ystride = c->d.ld;
+    }
     const int prof = c->profile_ipm && c->pev_n + 2 <= c->pev_cap;
     if (prof) {
         a.flops = c->d_flops;
+        a.flops_mx = c->d_flops + 1;
         hipEventRecord(c->pev[c->pev_n], c->stream);
     }
     if (bqb_check(bqb_launch_ipm_batch(&a, c->d.nodes, c->tile, c->stream), "ipm launch")) return -1;
+    if (a.mix_slices &&
+        bqb_check(bqb_launch_ipm_retry(&a, c->d.nodes, c->tile, c->p.mask2, &c->p.st[0].flags,
+                                       (int)(sizeof(NodeState) / sizeof(int)), c->stream), "ipm retry launch"))
+        return -1;
     if (prof) {
         hipEventRecord(c->pev[c->pev_n + 1], c->stream);
         c->pev_n += 2;
@@ -351,6 +368,8 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     case 107: return bqb_check(bqb_launch_opbt(d, p, arg, s), "opbt launch");
     case 108: return bqb_check(bqb_launch_bundle_qp(d, p, arg, s), "qp launch");
     case 109: return bqb_check(bqb_launch_bundle_step(d, p, s), "step launch");
+    case 110: return bqb_check(bqb_launch_cert_node(d, p, 0, s), "certificate launch");
+    case 111: return bqb_check(bqb_launch_cert_node(d, p, 1, s), "certificate launch");
     default:
         if (what >= 0 && what <= PH_BUNDLE_NEXT)
             return bqb_check(bqb_launch_control(d, p, what, arg, s), "control launch");
@@ -566,6 +585,7 @@ int bqb_ctx_advance(bqb_ctx *c)
         RUN(PH_SETUP, 0);
         RUN(100, 0);             /* basic IPM: L -> X */
         RUN(PH_POST_BASIC, 0);
+        if (c->d.certify) RUN(110, 0);   /* certified basic bound */
         RUN(102, 0);             /* heuristic */
         RUN(PH_CHECK_BASIC, 0);
         RUN(103, 0);             /* first triangle separation */
@@ -608,6 +628,7 @@ int bqb_ctx_advance(bqb_ctx *c)
             RUN(107, 1);         /* L0 = L - B'(gamma_test) */
             RUN(101, 0);         /* IPM: L0 -> X_test */
             RUN(109, 0);         /* serious / null step */
+            if (c->d.certify) RUN(111, 0);   /* certify the new center */
         }
         RUN(PH_OUTER_A, 0);
         RUN(102, 0);             /* heuristic */
@@ -1084,11 +1105,33 @@ int bqb_ctx_profile_ipm(bqb_ctx *c, int max_launches)
     }
     c->pev_n = 0;
     c->profile_ipm = 1;
-    return bqb_check(hipMemset(c->d_flops, 0, sizeof(unsigned long long)), "hipMemset flops");
+    return bqb_check(hipMemset(c->d_flops, 0, 2 * sizeof(unsigned long long)), "hipMemset flops");
 }
 
 /* IPM launches timed since profiling started: count, summed duration (ms)
  * and algorithmic flops. */
+int bqb_ctx_set_precision(bqb_ctx *c, int slices, int certify)
+{
+    if (slices < 0 || slices == 1 || slices > BQB_MIX_MAX_SLICES) {
+        bqb_set_error("slices = %d: 0 (FP64 products) or 2..%d", slices, BQB_MIX_MAX_SLICES);
+        return -1;
+    }
+    if (bqb_ctx_busy(c) > 0) { bqb_set_error("set_precision: nodes are in flight"); return -1; }
+    c->d.mix_slices = slices;
+    c->d.certify = (certify || slices > 0) ? 1 : 0;
+    c->d.work_slabs = BQB_IPM_WORK_SLABS(slices);
+    return 0;
+}
+
+int bqb_ctx_ipm_flops_mx(bqb_ctx *c, double *flops_mx)
+{
+    unsigned long long f[2] = {0, 0};
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (bqb_check(hipMemcpy(f, c->d_flops, sizeof f, hipMemcpyDeviceToHost), "D2H flops")) return -1;
+    *flops_mx = (double)f[1];
+    return 0;
+}
+
 int bqb_ctx_ipm_stats(bqb_ctx *c, int *launches, double *ms_total, double *flops)
 {
     if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;

@@ -6,6 +6,7 @@
 
 #include "bqb_dist.h"
 #include "bqb_kernels.h"
+#include "bqb_mixed.h"
 
 namespace bqb {
 
@@ -29,6 +30,17 @@ __global__ __launch_bounds__(P * P) void dist_selftest_kernel(int op, int n, int
     if (t.tid == 0) *flag = ok ? 1 : 0;
 }
 
+// C = A B (n x n, column-major, n <= 256) through the sliced bf16
+// matrix-core product of the mixed-precision IPM; S holds the slices.
+__global__ __launch_bounds__(256) void mx_selftest_kernel(int n, int s, const double *A, const double *B, double *C,
+                                                          unsigned short *S)
+{
+    __shared__ int e[2 * 256];
+    const auto la = [&](int i, int k) { return A[i + (size_t)k * n]; };
+    const auto lb = [&](int j, int k) { return B[k + (size_t)j * n]; };
+    mx_product<256>(la, lb, n, s, S, e, C, n);
+}
+
 // Plain streaming copy (16-byte vectors, grid-stride): the calibration kernel
 // for the HBM byte counters (its algorithmic traffic is exactly 2 x bytes).
 __global__ __launch_bounds__(256) void copy_probe_kernel(const double2 *__restrict__ src,
@@ -46,6 +58,13 @@ extern "C" hipError_t bqb_launch_copy_probe(const void *src, void *dst, size_t b
     for (int r = 0; r < reps; ++r)
         hipLaunchKernelGGL(bqb::copy_probe_kernel, dim3(4096), dim3(256), 0, 0, (const double2 *)src,
                            (double2 *)dst, n2);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bqb_launch_mx_selftest(int n, int s, const double *A, const double *B, double *C,
+                                             unsigned short *S)
+{
+    hipLaunchKernelGGL(bqb::mx_selftest_kernel, dim3(1), dim3(256), 0, 0, n, s, A, B, C, S);
     return hipGetLastError();
 }
 

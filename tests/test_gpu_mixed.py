@@ -1,0 +1,224 @@
+"""The mixed-precision form on the GPU: the IPM products on the bf16 matrix
+cores (sliced operands, exact FP32 accumulation, FP64 combination), FP64
+inversions and Cholesky line searches, and certified bounds computed with
+directed rounding.
+
+What is compared, and at what tolerance:
+* the sliced product itself against an FP64 product, within its a priori
+  error bound (4s + 1) n 2^-8s max|A(i,:)| max|B(:,j)| (plus FP64 rounding);
+* the mixed IPM's phi (4 slices, the setting the solver uses) against the
+  CPU oracle's FP64 phi on identical inputs, within 1e-6 relative (the
+  per-node bound criterion), per call (3 slices leave up to 2e-6 and can
+  stall a line search, which the FP64 re-solve then covers);
+* certified bounds: never below the oracle's FP64 value minus 1e-6
+  relative, at most 1e-7 relative above the uncertified value, never below
+  the brute-force maximum cut (n <= 22), and, for n <= 10, exactly valid:
+  Diag(y) - L + ((phi_cert - e'y)/n) I is positive semidefinite in rational
+  arithmetic;
+* whole nodes of G(250, 0.1) with weights +-1..+-10 (three outer rounds,
+  triangle cuts): each node's certified mixed bound against the FP64
+  oracle's, error stated, within 1e-4 relative (the trajectory's own
+  sensitivity, see test_gpu_bound.py) and never below the node's cut;
+* branch and bound with certified mixed bounds proves the reference's
+  optima exactly."""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from fixture_util import brute_maxcut, fixture_value
+from parallel_biqbin_maxcut_amd import api, instances as I
+
+pytestmark = pytest.mark.gpu
+
+
+def _subs(n0, p, weights, seed, count, nfix):
+    L = O.build_L(I.gnp_adj(n0, p, seed, weights=weights))
+    xf, sx = I.random_fixings(n0, nfix, count, seed + 1)
+    return [O.create_subproblem(L, xf[b], sx[b])[0] for b in range(count)]
+
+
+@pytest.mark.parametrize("n,s", [(16, 3), (40, 2), (100, 3), (151, 3), (250, 3), (256, 4)])
+def test_sliced_product_within_its_error_bound(n, s):
+    rng = np.random.default_rng(n + s)
+    A = rng.normal(size=(n, n)) * np.exp(rng.normal(size=(n, 1)) * 3)
+    B = rng.normal(size=(n, n)) * np.exp(rng.normal(size=(1, n)) * 3)
+    A[rng.random((n, n)) < 0.2] = 0.0
+    C = api.mx_product(A, B, slices=s)
+    ref = A @ B
+    scale = np.abs(A).max(axis=1)[:, None] * np.abs(B).max(axis=0)[None, :]
+    bound = ((4 * s + 1) * n * 2.0 ** (-8 * s) + 4 * n * 2.0 ** -53) * scale
+    err = np.abs(C - ref)
+    print(f"n {n} s {s}: max err/bound {np.max(err / bound):.3e}", flush=True)
+    assert np.all(err <= bound + 1e-300)
+
+
+@pytest.mark.parametrize("slices", [4])
+def test_mixed_ipm_phi_matches_fp64_oracle(slices):
+    Ls = (_subs(100, 0.3, "pm1", 3, 8, 40) + _subs(151, 0.5, "int10", 5, 3, 40)
+          + _subs(250, 0.1, "int10", 31, 3, 40) + _subs(30, 0.5, "unit", 2, 3, 5))
+    r = api.ipm_batch_mixed(Ls, slices=slices)
+    worst = 0.0
+    for b, L in enumerate(Ls):
+        _, _, phi, st, _ = O.ipm(L)
+        assert r["status"][b] == 0 and st.status == 0
+        rel = abs(r["phi"][b] - phi) / abs(phi)
+        worst = max(worst, rel)
+        print(f"n {L.shape[0]}: phi rel err {rel:.2e}, iters {r['iters'][b]} vs {st.iters}", flush=True)
+        assert rel <= 1e-6, (b, L.shape[0], r["phi"][b], phi)
+        cert = r["phi_cert"][b]
+        assert cert >= phi * (1 - 1e-6)
+        assert cert >= r["phi"][b]
+        assert cert - r["phi"][b] <= 1e-7 * abs(r["phi"][b]), (cert, r["phi"][b])
+    print(f"slices {slices}: worst phi error {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("slices", [0, 4])
+def test_certified_phi_above_brute_force(slices):
+    for seed in range(6):
+        adj = I.gnp_adj(18, 0.5, seed, weights="int10")
+        L, _ = O.create_subproblem(O.build_L(adj), np.zeros(17, np.int32), np.zeros(17, np.int32))
+        r = api.ipm_batch_mixed([L], slices=slices)
+        assert r["status"][0] == 0
+        assert r["phi_cert"][0] >= brute_maxcut(adj)
+
+
+def _psd_exact(M):
+    """Exact positive semidefiniteness of a symmetric rational matrix by
+    symmetric Gaussian elimination (LDL')."""
+    A = [row[:] for row in M]
+    n = len(A)
+    for k in range(n):
+        piv = A[k][k]
+        if piv < 0:
+            return False
+        if piv == 0:
+            if any(A[k][j] != 0 for j in range(k + 1, n)):
+                return False
+            continue
+        for i in range(k + 1, n):
+            f = A[i][k] / piv
+            if f != 0:
+                for j in range(k + 1, n):
+                    A[i][j] -= f * A[k][j]
+    return True
+
+
+@pytest.mark.parametrize("slices", [0, 4])
+def test_certificate_is_exactly_valid_small(slices):
+    """Diag(y) - L + ((phi_cert - e'y)/n) I is psd in exact arithmetic, with
+    y and phi_cert the doubles the GPU returned and L exact."""
+    Ls = _subs(10, 0.6, "int10", 7, 4, 2)
+    r = api.ipm_batch_mixed(Ls, slices=slices)
+    for b, L in enumerate(Ls):
+        n = L.shape[0]
+        assert r["status"][b] == 0
+        y = [Fraction(v) for v in r["y"][b]]
+        sh = (Fraction(r["phi_cert"][b]) - sum(y)) / n
+        Z = [[(y[i] if i == j else 0) - Fraction(L[i, j]) + (sh if i == j else 0) for j in range(n)]
+             for i in range(n)]
+        assert _psd_exact(Z), b
+
+
+def _bound_both(SP, P, xf, sx, seeds, slices):
+    n0 = SP.shape[0]
+    ctx = api.BoundContext(SP, params=P, max_nodes=xf.shape[0])
+    ctx.set_precision(slices=slices, certify=1)
+    r = ctx.bound(xf, sx, LB=0.0, diff=0.0, is_root=0, seeds=seeds)
+    ctx.close()
+    d = {f: getattr(P, f) for f, _ in api.BiqBinParameters._fields_}
+    if d["adjust_TriIneq"]:
+        d["TriIneq"] = 10 * n0
+    outs = [O.sdpbound(SP, xf[b], sx[b], 0.0, 0.0, 0, int(seeds[b]), O.default_params(**d))[0]
+            for b in range(xf.shape[0])]
+    return r, outs
+
+
+@pytest.mark.parametrize("slices", [4])
+def test_cfg3_nodes_mixed_certified_vs_fp64_oracle(slices):
+    """G(250, 0.1), weights +-1..+-10: the root and three nodes with 5-20
+    fixings, three outer rounds with triangle cuts, mixed products and
+    certified bounds, against the FP64 oracle.  No node may need the FP64
+    re-solve here."""
+    n0 = 250
+    SP = O.build_L(I.gnp_adj(n0, 0.1, 3, weights="int10"))
+    P = api.make_params(include_Pent=0, include_Hepta=0, max_outer_iter=3, extra_iter=0)
+    xf, sx = I.random_fixings(n0, 20, 4, 9)
+    xf[0] = 0
+    sx[0] = 0
+    seeds = np.arange(5, 9, dtype=np.uint64)
+    r, outs = _bound_both(SP, P, xf, sx, seeds, slices)
+    for b, o in enumerate(outs):
+        print(f"node {b}: status {r['status'][b]} / oracle {o.status}", flush=True)
+        assert r["status"][b] == 0 and o.status == 0
+        rel = (r["bound"][b] - o.bound) / abs(o.bound)
+        print(f"node {b}: n {r['stats'][b][7]}, mixed certified {r['bound'][b]:.6f}, FP64 oracle {o.bound:.6f}, "
+              f"rel {rel:+.2e}", flush=True)
+        assert abs(rel) <= 1e-4
+        assert r["bound"][b] >= r["LB"][b]
+        assert not r["stats"][b][8] & api.FLAG_FP64_RETRY
+
+
+def test_certified_fp64_bound_tracks_plain_bound():
+    """FP64 products with certification: same trajectory as the plain
+    bound; each certified bound at most 1e-7 relative above it."""
+    n0 = 60
+    SP = O.build_L(I.gnp_adj(n0, 0.5, 1, weights="unit"))
+    P = api.make_params(include_Pent=0, include_Hepta=0, max_outer_iter=3, extra_iter=0)
+    xf, sx = I.random_fixings(n0, 10, 6, 2)
+    seeds = np.arange(1, 7, dtype=np.uint64)
+    ctx = api.BoundContext(SP, params=P, max_nodes=6)
+    plain = ctx.bound(xf, sx, LB=0.0, diff=0.0, is_root=0, seeds=seeds)
+    ctx.set_precision(slices=0, certify=1)
+    cert = ctx.bound(xf, sx, LB=0.0, diff=0.0, is_root=0, seeds=seeds)
+    ctx.close()
+    for b in range(6):
+        assert plain["status"][b] == 0 and cert["status"][b] == 0
+        d = cert["bound"][b] - plain["bound"][b]
+        assert 0.0 <= d <= 1e-7 * abs(plain["bound"][b]), (b, cert["bound"][b], plain["bound"][b])
+
+
+@pytest.mark.parametrize("name", ["g05_60.0", "g05_80.0"])
+def test_mixed_certified_bab_proves_fixture_optimum(golden, name):
+    path = os.path.join(golden, "rudy", name)
+    SP = api.maxcut_L(api.read_rudy(path))
+    P = api.make_params()
+    P.TriIneq = 10 * SP.shape[0]
+    st, sol = api.solve(SP, P, seed=2020, batch=32, slices=4)
+    print(f"{name}: mixed certified B&B {st.nodes} nodes, {st.seconds:.2f} s", flush=True)
+    assert st.status == 0
+    assert st.best == fixture_value(path + "-expected_output")
+    assert api.evaluate_solution(SP, sol) == st.best
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_mixed_certified_bab_brute_force(seed):
+    adj = I.gnp_adj(20, 0.5, seed, weights="int10")
+    SP = api.maxcut_L(adj)
+    P = api.make_params()
+    P.TriIneq = 200
+    st, sol = api.solve(SP, P, seed=seed, batch=8, slices=4)
+    assert st.status == 0 and st.best == brute_maxcut(adj)
+
+
+def test_three_slices_fall_back_to_fp64_where_needed():
+    """With 3 slices the products are accurate to about 1e-7 of the line
+    maxima, which can stall a line search near the boundary of the cone
+    (node 1 of this batch); such solves are repeated with FP64 products and
+    flagged, and every node still ends with status 0 and a bound within
+    1e-4 of the FP64 oracle."""
+    n0 = 250
+    SP = O.build_L(I.gnp_adj(n0, 0.1, 3, weights="int10"))
+    P = api.make_params(include_Pent=0, include_Hepta=0, max_outer_iter=3, extra_iter=0)
+    xf, sx = I.random_fixings(n0, 20, 4, 9)
+    xf[0] = 0
+    sx[0] = 0
+    seeds = np.arange(5, 9, dtype=np.uint64)
+    r, outs = _bound_both(SP, P, xf, sx, seeds, 3)
+    for b, o in enumerate(outs):
+        retried = bool(r["stats"][b][8] & api.FLAG_FP64_RETRY)
+        print(f"node {b}: status {r['status'][b]}, FP64 re-solve {retried}", flush=True)
+        assert r["status"][b] == 0 and o.status == 0
+        assert abs(r["bound"][b] - o.bound) <= 1e-4 * abs(o.bound)

@@ -28,7 +28,16 @@ groups in flight a launch's duration includes the time its workgroups wait
 for CUs held by other kernels, so the object also gives the same flops over
 the timed wall time (aggregate_achieved).  The
 cpu_baseline times the CPU oracle (the FP64 restatement of the same
-algorithm) on a bounded sample of the same nodes with one thread per core.
+algorithm) on a bounded sample of the same nodes with one thread per core;
+port_over_reference_root_time ties it to the reference binary (measured on
+the same host type, one core each).
+
+Besides the headline, config carries: time_to_proof (G(60, 0.5) branch and
+bound to the proven optimum, one node at a time, 64 slots, and 64 slots with
+the mixed-precision certified bounds), cfg3_mixed (G(250, 0.1), FP64 against
+bf16 matrix-core products with certified bounds, with the roofline of the
+mixed IPM against the bf16 peak) and cfg4_qubo (QUBO 150 as 151-vertex
+Max-Cut, 256 nodes per launch).
 """
 import argparse
 import json
@@ -92,21 +101,29 @@ def time_to_proof(api, I, seeds, cpu_seeds):
     P = api.make_params(include_Pent=0, include_Hepta=0)
     out = {"workload": f"G(60, 0.5) unit weights, triangles only, seeds {seeds[0]}-{seeds[-1]}",
            "optimum": [], "nodes_batch1": [], "seconds_batch1": [], "seconds_batch64": []}
+    out["seconds_batch64_mixed"] = []
     Ls = []
+    nodes64 = 0
     for sd in seeds:
         L = api.maxcut_L(I.gnp_adj(60, 0.5, sd, weights="unit"))
         Ls.append(L)
         st1, _ = api.solve(L, params=P, seed=2020, batch=1)
         st64, _ = api.solve(L, params=P, seed=2020, batch=64)
-        if st1.status or st64.status or st1.best != st64.best:
-            raise SystemExit(f"time-to-proof instance {sd}: status {st1.status}/{st64.status}, "
-                             f"optimum {st1.best} vs {st64.best}")
+        stm, _ = api.solve(L, params=P, seed=2020, batch=64, slices=4)
+        if st1.status or st64.status or stm.status or not (st1.best == st64.best == stm.best):
+            raise SystemExit(f"time-to-proof instance {sd}: status {st1.status}/{st64.status}/{stm.status}, "
+                             f"optimum {st1.best} / {st64.best} / {stm.best}")
         out["optimum"].append(st1.best)
         out["nodes_batch1"].append(int(st1.nodes))
         out["seconds_batch1"].append(round(st1.seconds, 4))
         out["seconds_batch64"].append(round(st64.seconds, 4))
+        out["seconds_batch64_mixed"].append(round(stm.seconds, 4))
+        nodes64 += int(st64.nodes)
     out["median_s_batch1"] = float(np.median(out["seconds_batch1"]))
     out["median_s_batch64"] = float(np.median(out["seconds_batch64"]))
+    out["median_s_batch64_mixed"] = float(np.median(out["seconds_batch64_mixed"]))
+    # bounds/s on the node stream of a real branch and bound (64 slots)
+    out["bab_node_bounds_per_s_batch64"] = round(nodes64 / sum(out["seconds_batch64"]), 2)
     if cpu_seeds:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
@@ -123,6 +140,99 @@ def time_to_proof(api, I, seeds, cpu_seeds):
         out["cpu_oracle_seconds_batch1"] = cs
         out["cpu_oracle_cores"] = 1
     return out
+
+
+# Root-node bound times of the CPU oracle (this build's FP64 port, one core)
+# against the reference binary (MKL, one core) measured on the same 8-core
+# Xeon host, same configurations (the reference's figures: 1.38 s cfg2-like,
+# 4.42 s QUBO 150 -> 151 vertices, 10.9 s G(250, 0.1)): the port is this many
+# times slower per node, so the reference's own CPU throughput is about
+# cpu_baseline.value times this ratio.
+PORT_OVER_REFERENCE = {"cfg2 G(100, 0.3) root": round(3.95 / 1.38, 2),
+                       "cfg4 QUBO 150 root": round(26.72 / 4.42, 2),
+                       "cfg3 G(250, 0.1) root": round(68.49 / 10.9, 2)}
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 matrix peak
+
+
+def mixed_section(api, I, seed, nodes=64):
+    """cfg3: Max-Cut on G(250, 0.1) with weights uniform in {-10..10} without 0:
+    the root bound, then `nodes` B&B nodes with 0-40 random fixings (sizes
+    210-250) bounded as one batch, with FP64 products and with the products
+    on the bf16 matrix cores (4 slices) and certified bounds.  The roofline
+    of the mixed IPM kernel: its bf16 matrix-core flops (2 n^3 per slice pair
+    and product) over the summed IPM launch time of the node batch."""
+    SP = api.maxcut_L(I.gnp_adj(250, 0.1, seed, weights="int10"))
+    P = api.make_params()
+    P.TriIneq = 2500                                  # adjust_TriIneq: 10 * n0
+    xf, sx = I.random_fixings(250, 40, nodes, seed + 1)
+    seeds = np.arange(1, nodes + 1, dtype=np.uint64)
+    z = np.zeros((1, 249), np.int32)
+    out = {"workload": f"G(250, 0.1) weights +-1..+-10, root bound then {nodes} nodes with 0-40 fixings "
+                       "(sizes 210-250) as one batch; reference params"}
+    res = {}
+    for mode, slices in (("fp64", 0), ("mixed", 4)):
+        ctx = api.BoundContext(SP, params=P, max_nodes=nodes)
+        ctx.set_precision(slices=slices, certify=1 if slices else 0)
+        t0 = time.perf_counter()
+        root = ctx.bound(z, z, LB=0.0, is_root=1, seeds=np.uint64(seed))
+        t_root = time.perf_counter() - t0
+        LB, diff = float(root["LB"][0]), float(root["diff"][0])
+        ctx.profile_ipm(max_launches=20000)
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds)
+        dt = time.perf_counter() - t0
+        launches, ipm_ms, fl64 = ctx.ipm_stats()
+        flmx = ctx.ipm_flops_mx()
+        ctx.profile_ipm(0)
+        ctx.close()
+        if np.any(r["status"]):
+            raise SystemExit(f"cfg3 {mode}: node failures {np.unique(r['status'])}")
+        res[mode] = (r, dt, ipm_ms, fl64, flmx, launches)
+        out[f"{mode}_root_s"] = round(t_root, 3)
+        out[f"{mode}_root_bound"] = round(float(root["bound"][0]), 6)
+        out[f"{mode}_bounds_per_s"] = round(nodes / dt, 3)
+        out[f"{mode}_ipm_ms"] = round(ipm_ms, 2)
+    bp, bm = res["fp64"][0]["bound"], res["mixed"][0]["bound"]
+    rel = np.abs(bm - bp) / np.abs(bp)
+    out["speedup_bounds_per_s"] = round(out["mixed_bounds_per_s"] / out["fp64_bounds_per_s"], 3)
+    out["speedup_root"] = round(out["fp64_root_s"] / out["mixed_root_s"], 3)
+    out["rel_bound_diff_mixed_vs_fp64"] = {"median": float(np.median(rel)), "max": float(rel.max())}
+    out["fp64_retries"] = int(np.sum(res["mixed"][0]["stats"][:, 8] & api.FLAG_FP64_RETRY > 0))
+    _, _, ipm_ms, fl64, flmx, launches = res["mixed"]
+    ach = flmx / (ipm_ms / 1e3) / 1e12 if ipm_ms > 0 else 0.0
+    out["roofline"] = {"kernel": "ipm_batch_kernel (products on bf16 matrix cores)", "bound": "mfma",
+                       "achieved": round(ach, 4), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(ach / BF16_PEAK_TFLOPS, 6), "traffic": None, "launches": launches,
+                       "fp64_achieved": round(fl64 / (ipm_ms / 1e3) / 1e12, 4) if ipm_ms > 0 else 0.0}
+    return out
+
+
+def qubo_section(api, I, seed, nodes=256):
+    """cfg4: dense symmetric integer QUBO, n = 150, entries uniform in
+    {-100..100} with 50 % density, as Max-Cut on 151 vertices
+    (biqbin_base.py:239-258); 256 B&B nodes with 0-40 fixings bounded per
+    launch (sizes 111-151), FP64, after the instance's root bound."""
+    SP = api.maxcut_L(I.qubo_to_maxcut_adj(I.random_qubo(150, 0.5, seed)))
+    P = api.make_params()
+    P.TriIneq = 1510
+    ctx = api.BoundContext(SP, params=P, max_nodes=nodes)
+    z = np.zeros((1, 150), np.int32)
+    t0 = time.perf_counter()
+    root = ctx.bound(z, z, LB=0.0, is_root=1, seeds=np.uint64(seed))
+    t_root = time.perf_counter() - t0
+    xf, sx = I.random_fixings(151, 40, nodes, seed + 1)
+    t0 = time.perf_counter()
+    r = ctx.bound(xf, sx, LB=float(root["LB"][0]), diff=float(root["diff"][0]), is_root=0,
+                  seeds=np.arange(1, nodes + 1, dtype=np.uint64))
+    dt = time.perf_counter() - t0
+    ctx.close()
+    if np.any(r["status"]):
+        raise SystemExit(f"cfg4: node failures {np.unique(r['status'])}")
+    return {"workload": "QUBO n=150, entries {-100..100} at 50% density -> Max-Cut on 151 vertices; "
+                        f"{nodes} nodes per launch with 0-40 fixings (sizes 111-151), FP64",
+            "root_s": round(t_root, 3), "root_bound": float(root["bound"][0]), "incumbent": float(root["LB"][0]),
+            "bounds_per_s": round(nodes / dt, 3), "mean_outer_rounds": round(float(r["stats"][:, 2].mean()), 3)}
 
 
 def traffic_per_launch():
@@ -147,6 +257,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-proof", action="store_true", help="skip the time-to-proof instances")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the cfg3 mixed-precision and cfg4 QUBO sections")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-batch comparison after the timed steps (profiling runs)")
     args = ap.parse_args()
@@ -250,6 +362,9 @@ def main():
     }
     if not args.no_proof:
         line["config"]["time_to_proof"] = time_to_proof(api, I, list(range(1, 21)), 0 if args.no_cpu else 3)
+    if not args.no_extra:
+        line["config"]["cfg3_mixed"] = mixed_section(api, I, args.seed)
+        line["config"]["cfg4_qubo"] = qubo_section(api, I, args.seed)
     if not args.no_cpu:
         sample = min(args.cpu_sample, args.per_step)
         cb, cbounds, cstatus = cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample)
@@ -257,6 +372,7 @@ def main():
         ok = cstatus == 0
         rel = np.abs(r["bound"][:sample][ok] - cbounds[ok]) / np.abs(cbounds[ok])
         line["cpu_baseline"]["max_rel_bound_diff_vs_gpu"] = float(rel.max()) if rel.size else None
+        line["cpu_baseline"]["port_over_reference_root_time"] = PORT_OVER_REFERENCE
     print(json.dumps(line), flush=True)
 
 

@@ -328,7 +328,8 @@ void bqb_ctx_trajectory_clear(bqb_ctx *c);
  * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP
  * (arg: an upper bound on every bundle size, 0 if unknown; a bound <= 16
  * selects the small-LDS QP kernel), 109 bundle step, 110/111 certificate of
- * the basic bound / of a new bundle center), run the whole bound,
+ * the basic bound / of a new bundle center, 112/113 learned model on the
+ * cuts of the first / of a round's separation), run the whole bound,
  * read results, and copy one node's
  * device buffer (ids documented in bqb_sdpbound.c) to or from the host. */
 int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
@@ -352,6 +353,46 @@ int bqb_ctx_set_precision(bqb_ctx *c, int slices, int certify);
 /* bf16 matrix-core flops counted since the profile started (2 n^3 per slice
  * pair and product). */
 int bqb_ctx_ipm_flops_mx(bqb_ctx *c, double *flops_mx);
+
+/* ---- learned cut model (no counterpart in the reference) ----------------
+ * A 3 x 256 MLP (32 -> 256 -> 256 -> 256 -> 2, ReLU) on the bf16 matrix
+ * cores with FP32 accumulation, trained by the library's own forward and
+ * backward kernels (Adam).  Input: the features of a cut (16 used, see
+ * parallel_biqbin_maxcut_amd/learn.py); outputs: log1p(gamma / 0.05) of the
+ * cut's final multiplier and the logit of the cut being active (gamma > 1e-5)
+ * at the end of the node.  Parameters: BQB_MLP_NPARAM floats, layout
+ * W1[256][32] b1 W2[256][256] b2 W3[256][256] b3 W4[16][256] b4[16] (rows 2..15
+ * of W4 and b4 unused, zero). */
+#define BQB_MLP_NPARAM 144144
+typedef struct bqb_mlp bqb_mlp;
+bqb_mlp *bqb_mlp_create(const float *params, uint64_t seed);   /* params NULL: random init */
+void bqb_mlp_destroy(bqb_mlp *m);
+int bqb_mlp_get_params(bqb_mlp *m, float *out);
+int bqb_mlp_set_params(bqb_mlp *m, const float *in);
+/* feat [M][32] FP32 (rounded to bf16 on upload), out [M][2] */
+int bqb_mlp_forward(bqb_mlp *m, int M, const float *feat, float *out);
+/* one forward + backward over all M rows: grad [BQB_MLP_NPARAM], mean loss */
+int bqb_mlp_gradient(bqb_mlp *m, int M, const float *feat, const float *target, float *grad, double *loss);
+/* Adam, `epochs` passes over the rows in shuffled batches; loss_out[epochs] */
+int bqb_mlp_train(bqb_mlp *m, int M, const float *feat, const float *target, int epochs, int batch, float lr,
+                  uint64_t seed, double *loss_out);
+/* Diagnostic: the features the bound computes for explicit cuts of one node
+ * (X, L n x n column-major; cut records as bqb_fct_eval); out [m][32]. */
+int bqb_cut_features(int n, const double *X, const double *L, int ntri, const int *tri, int npent,
+                     const int *pent, int nhept, const int *hept, float round, float gap, float *out);
+/* Use the model in a context's bounds: after every separation the new cuts
+ * (src/cutting_planes.c:201, 392, 588 start them at 0) start from the
+ * predicted multiplier times the predicted probability of being active, new
+ * cuts predicted active with probability below `drop` are left out (0:
+ * none), and the round's rebuild evaluation becomes the value f of the
+ * warm-started center.  NULL detaches.  The model must outlive its use by
+ * the context. */
+int bqb_ctx_set_model(bqb_ctx *c, const bqb_mlp *model, float drop);
+/* Training data: log the features of a quarter of the new cuts of every
+ * separation (up to capacity records; 0 stops); records: feat [k][16],
+ * key [k][12] = cut size, type, 7 vertices (padded), round, node seed (2 ints). */
+int bqb_ctx_learn_record(bqb_ctx *c, int capacity);
+int bqb_ctx_learn_records(bqb_ctx *c, int *count, float *feat, int *key);
 
 /* IPM launch profile (no counterpart in the reference; for benchmarks):
  * start with max_launches > 0 (event pair per IPM launch, device flop

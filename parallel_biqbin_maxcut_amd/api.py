@@ -29,6 +29,8 @@ def declare(L):
     L.bqb_ipm_batch_noalloc.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
     L.bqb_ipm_batch_noalloc.restype = C.c_int
     _declare_ctx(L)
+    from . import learn
+    learn.declare(L)
 
 
 def _d(a):

@@ -74,6 +74,7 @@ typedef struct BatchDims {
     int mix_slices;     /* IPM products: 0 FP64, 2..4 bf16 slices on the matrix cores */
     int certify;        /* prune on certified bounds (f_cert) instead of f */
     int work_slabs;     /* ld x ld slabs of IPM work per node */
+    int learned;        /* new cuts start from the cut model's multipliers */
 } BatchDims;
 
 /* Device pointers of a batch (all arrays are per node at the stated stride). */

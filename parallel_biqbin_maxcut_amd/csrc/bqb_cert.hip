@@ -244,7 +244,9 @@ __global__ __launch_bounds__(CE) void cert_ipm_kernel(const int *nvec, const dou
 // ---- the node bound: certificate of the point just evaluated ----
 // mode 0: basic bound (no cuts; phi of the IPM on L, y in ytest);
 // mode 1: after a bundle step that was serious (gamma = gamma_test, the cut
-// lists and gather lists of the node, y of the IPM on L - B'gamma_test).
+// lists and gather lists of the node, y of the IPM on L - B'gamma_test);
+// mode 2: after the rebuild of a round that made the warm-started center
+// gamma the new f (y of the IPM on L - B'gamma).
 // The certified value replaces NodeState.f_cert.
 __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p, int mode)
 {
@@ -256,6 +258,7 @@ __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p,
     const NodeState s = *sp;
     if (!s.active || s.status) return;
     if (mode == 1 && !s.serious) return;
+    if (mode == 2 && (s.stage != BQB_STAGE_LOOP || s.rphase != 0)) return;
     const int n = s.n;
     if (n < 1 || n > d.ld) return;
     const size_t slab = (size_t)d.ld * d.ld;
@@ -264,9 +267,9 @@ __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p,
     CutSource cs;
     const CutSource *csp = nullptr;
     double sg = 0.0;
-    if (mode == 1) {
+    if (mode >= 1) {
         const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
-        const double *gam = p.gtest + (size_t)node * d.max_m;
+        const double *gam = (mode == 1 ? p.gtest : p.gamma) + (size_t)node * d.max_m;
         cs.ptr = p.csr_ptr + (size_t)node * (d.npairs + 1);
         cs.lst = p.csr_cut + (size_t)node * d.max_contrib;
         cs.gam = gam;

@@ -431,7 +431,17 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             Xdst[e] = Xt[e];
         }
         const double Fb = dot_LX(p.L + node * slab, Xt, s.n, d.ld, red);
+        // With warm-started multipliers the new stability center gamma is no
+        // longer (nearly) the old one, so its value f(gamma) = phi + sum(gamma)
+        // from this evaluation becomes f (any gamma >= 0 gives a valid bound).
+        double sg = 0.0;
+        if (d.learned) {
+            const double *gam = p.gamma + (size_t)node * d.max_m;
+            for (int c = threadIdx.x; c < m; c += CT) sg += gam[c];
+            sg = block_sum(sg, red);
+        }
         if (threadIdx.x == 0) {
+            if (d.learned) sp->f = p.phi[node] + sg;
             p.F[(size_t)node * d.max_bundle + s.bundle] = Fb;
             sp->bundle = s.bundle + 1;
             if (sp->bundle > s.max_bundle_seen) sp->max_bundle_seen = sp->bundle;

@@ -37,6 +37,9 @@ hipError_t bqb_launch_heur(const BatchDims *d, const BatchPtrs *p, int tile, int
 hipError_t bqb_launch_bundle_qp(const BatchDims *d, const BatchPtrs *p, int kbound, hipStream_t s);
 hipError_t bqb_launch_bundle_step(const BatchDims *d, const BatchPtrs *p, hipStream_t s);
 hipError_t bqb_launch_cert_node(const BatchDims *d, const BatchPtrs *p, int mode, hipStream_t s);
+hipError_t bqb_launch_learn(const BatchDims *d, const BatchPtrs *p, const bqb_mlp *m, float drop, int *rec_count,
+                            int rec_cap, float *rec_feat, int *rec_key, int *idx, unsigned char *keep,
+                            int after_round, hipStream_t s);
 
 struct bqb_ctx {
     BatchDims d;
@@ -82,6 +85,15 @@ struct bqb_ctx {
     int *tr_n;                 /* [max_nodes] rounds logged */
     bqb_trajectory *traj;      /* each record owns one block (its round_f) */
     int ntraj, traj_cap;
+    /* learned cut model: prediction (model) or recording of new cuts */
+    const bqb_mlp *model;
+    float drop;
+    int rec_cap;
+    int *d_rec_count;          /* [1] */
+    float *d_rec_feat;         /* [rec_cap][16] */
+    int *d_rec_key;            /* [rec_cap][12] */
+    int *d_lidx;               /* [max_nodes][max_m] */
+    unsigned char *d_lkeep;    /* [max_nodes][max_m] */
 };
 
 struct done_node {
@@ -301,6 +313,8 @@ void bqb_ctx_destroy(bqb_ctx *c)
 {
     if (!c) return;
     hipStreamSynchronize(c->stream);
+    if (c->d_rec_count) hipFree(c->d_rec_count);
+    if (c->d_lidx) hipFree(c->d_lidx);
     free_events(c);
     free_host(c);
     hipFree(c->arena);
@@ -370,6 +384,12 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     case 109: return bqb_check(bqb_launch_bundle_step(d, p, s), "step launch");
     case 110: return bqb_check(bqb_launch_cert_node(d, p, 0, s), "certificate launch");
     case 111: return bqb_check(bqb_launch_cert_node(d, p, 1, s), "certificate launch");
+    case 114: return bqb_check(bqb_launch_cert_node(d, p, 2, s), "certificate launch");
+    case 112: case 113:
+        if (!c->model && !c->rec_cap) return 0;
+        return bqb_check(bqb_launch_learn(d, p, c->model, c->drop, c->rec_cap ? c->d_rec_count : NULL, c->rec_cap,
+                                          c->d_rec_feat, c->d_rec_key, c->d_lidx, c->d_lkeep, what == 113, s),
+                         "learn launch");
     default:
         if (what >= 0 && what <= PH_BUNDLE_NEXT)
             return bqb_check(bqb_launch_control(d, p, what, arg, s), "control launch");
@@ -589,6 +609,7 @@ int bqb_ctx_advance(bqb_ctx *c)
         RUN(102, 0);             /* heuristic */
         RUN(PH_CHECK_BASIC, 0);
         RUN(103, 0);             /* first triangle separation */
+        RUN(112, 0);             /* learned warm start / ranking of the new cuts */
         RUN(PH_BUNDLE_INIT, 0);
         RUN(106, 0);             /* op_Bt gather lists */
         c->loaded = 0;
@@ -638,11 +659,13 @@ int bqb_ctx_advance(bqb_ctx *c)
         RUN(104, 0);             /* pentagonal */
         RUN(PH_SEP_HEPT, 0);
         RUN(105, 0);             /* heptagonal */
+        RUN(113, 0);             /* learned warm start / ranking of the new cuts */
         RUN(PH_OUTER_C, 0);
         RUN(106, 0);             /* new gather lists */
         RUN(107, 0);             /* L0 = L - B'(gamma) */
         RUN(101, 0);             /* IPM: L0 -> X_test */
         RUN(PH_REBUILD, 0);
+        if (c->d.learned && c->d.certify) RUN(114, 0);   /* certify the warm-started center */
         if (fetch_states(c)) return -1;
     }
     if (c->capture) log_rounds(c);
@@ -1110,6 +1133,56 @@ int bqb_ctx_profile_ipm(bqb_ctx *c, int max_launches)
 
 /* IPM launches timed since profiling started: count, summed duration (ms)
  * and algorithmic flops. */
+static int learn_scratch(bqb_ctx *c)
+{
+    if (c->d_lidx) return 0;
+    const size_t cells = (size_t)c->max_nodes * c->d.max_m;
+    if (bqb_check(hipMalloc((void **)&c->d_lidx, cells * (sizeof(int) + 1)), "hipMalloc learn scratch")) return -1;
+    c->d_lkeep = (unsigned char *)(c->d_lidx + cells);
+    return 0;
+}
+
+int bqb_ctx_set_model(bqb_ctx *c, const bqb_mlp *model, float drop)
+{
+    if (bqb_ctx_busy(c) > 0) { bqb_set_error("set_model: nodes are in flight"); return -1; }
+    if (model && learn_scratch(c)) return -1;
+    c->model = model;
+    c->drop = model ? drop : 0.f;
+    c->d.learned = (model && !c->rec_cap) ? 1 : 0;
+    return 0;
+}
+
+int bqb_ctx_learn_record(bqb_ctx *c, int capacity)
+{
+    if (bqb_ctx_busy(c) > 0) { bqb_set_error("learn_record: nodes are in flight"); return -1; }
+    if (c->d_rec_count) { hipFree(c->d_rec_count); c->d_rec_count = NULL; }
+    c->rec_cap = 0;
+    if (capacity <= 0) return 0;
+    if (learn_scratch(c)) return -1;
+    const size_t bytes = 256 + (size_t)capacity * (16 * sizeof(float) + 12 * sizeof(int));
+    if (bqb_check(hipMalloc((void **)&c->d_rec_count, bytes), "hipMalloc records")) return -1;
+    c->d_rec_feat = (float *)((char *)c->d_rec_count + 256);
+    c->d_rec_key = (int *)(c->d_rec_feat + (size_t)capacity * 16);
+    if (bqb_check(hipMemset(c->d_rec_count, 0, sizeof(int)), "hipMemset records")) return -1;
+    c->rec_cap = capacity;
+    return 0;
+}
+
+int bqb_ctx_learn_records(bqb_ctx *c, int *count, float *feat, int *key)
+{
+    int k = 0;
+    if (!c->rec_cap) { *count = 0; return 0; }
+    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (bqb_check(hipMemcpy(&k, c->d_rec_count, sizeof(int), hipMemcpyDeviceToHost), "D2H count")) return -1;
+    if (k > c->rec_cap) k = c->rec_cap;
+    *count = k;
+    if (k && feat && bqb_check(hipMemcpy(feat, c->d_rec_feat, sizeof(float) * 16 * (size_t)k, hipMemcpyDeviceToHost), "D2H records"))
+        return -1;
+    if (k && key && bqb_check(hipMemcpy(key, c->d_rec_key, sizeof(int) * 12 * (size_t)k, hipMemcpyDeviceToHost), "D2H records"))
+        return -1;
+    return 0;
+}
+
 int bqb_ctx_set_precision(bqb_ctx *c, int slices, int certify)
 {
     if (slices < 0 || slices == 1 || slices > BQB_MIX_MAX_SLICES) {

@@ -102,6 +102,10 @@ def time_to_proof(api, I, seeds, cpu_seeds):
     out = {"workload": f"G(60, 0.5) unit weights, triangles only, seeds {seeds[0]}-{seeds[-1]}",
            "optimum": [], "nodes_batch1": [], "seconds_batch1": [], "seconds_batch64": []}
     out["seconds_batch64_mixed"] = []
+    out["seconds_batch64_mixed_learned"] = []
+    from parallel_biqbin_maxcut_amd import learn
+    model = learn.MLP(params=np.load(os.path.join(ROOT, "parallel_biqbin_maxcut_amd", "models", "cut_model.npz"),
+                                     allow_pickle=False)["params"])
     Ls = []
     nodes64 = 0
     for sd in seeds:
@@ -110,18 +114,26 @@ def time_to_proof(api, I, seeds, cpu_seeds):
         st1, _ = api.solve(L, params=P, seed=2020, batch=1)
         st64, _ = api.solve(L, params=P, seed=2020, batch=64)
         stm, _ = api.solve(L, params=P, seed=2020, batch=64, slices=4)
-        if st1.status or st64.status or stm.status or not (st1.best == st64.best == stm.best):
-            raise SystemExit(f"time-to-proof instance {sd}: status {st1.status}/{st64.status}/{stm.status}, "
-                             f"optimum {st1.best} / {st64.best} / {stm.best}")
+        stl, _ = api.solve(L, params=P, seed=2020, batch=64, slices=4, model=model)
+        if st1.status or st64.status or stm.status or stl.status or not (st1.best == st64.best == stm.best == stl.best):
+            raise SystemExit(f"time-to-proof instance {sd}: status {st1.status}/{st64.status}/{stm.status}/"
+                             f"{stl.status}, optimum {st1.best} / {st64.best} / {stm.best} / {stl.best}")
         out["optimum"].append(st1.best)
         out["nodes_batch1"].append(int(st1.nodes))
         out["seconds_batch1"].append(round(st1.seconds, 4))
         out["seconds_batch64"].append(round(st64.seconds, 4))
         out["seconds_batch64_mixed"].append(round(stm.seconds, 4))
+        out["seconds_batch64_mixed_learned"].append(round(stl.seconds, 4))
         nodes64 += int(st64.nodes)
+    model.close()
     out["median_s_batch1"] = float(np.median(out["seconds_batch1"]))
     out["median_s_batch64"] = float(np.median(out["seconds_batch64"]))
     out["median_s_batch64_mixed"] = float(np.median(out["seconds_batch64_mixed"]))
+    out["median_s_batch64_mixed_learned"] = float(np.median(out["seconds_batch64_mixed_learned"]))
+    # the headline comparison: learned + batched + mixed against the plain
+    # FP64 form one node at a time, same instances, same proven optima
+    out["speedup_learned_batched_mixed_vs_plain"] = round(
+        sum(out["seconds_batch1"]) / sum(out["seconds_batch64_mixed_learned"]), 3)
     # bounds/s on the node stream of a real branch and bound (64 slots)
     out["bab_node_bounds_per_s_batch64"] = round(nodes64 / sum(out["seconds_batch64"]), 2)
     if cpu_seeds:

@@ -177,11 +177,14 @@ int bqb_solve(int n0, const double *SP_L, const BiqBinParameters *params, uint64
  * branched. */
 int bqb_solve_cap(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
                   int max_bundle, int *sol, bqb_solve_stats *stats);
-/* The same with the precision of the bounds (bqb_ctx_set_precision):
+/* The same with the precision of the bounds (bqb_ctx_set_precision:
  * slices 2..4 bound on the bf16 matrix cores with certified bounds,
- * certify = 1 certifies FP64 bounds. */
+ * certify = 1 certifies FP64 bounds) and a learned cut model
+ * (bqb_ctx_set_model; NULL: none). */
+typedef struct bqb_mlp bqb_mlp;
 int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
-                 int max_bundle, int slices, int certify, int *sol, bqb_solve_stats *stats);
+                 int max_bundle, int slices, int certify, const bqb_mlp *model, float drop, int *sol,
+                 bqb_solve_stats *stats);
 
 /* ---- batch: interior-point solve of the basic SDP ---------------------- */
 /* Solves max <L_b, X_b> s.t. diag(X_b) = e, X_b psd for `nodes` independent
@@ -341,6 +344,18 @@ int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, doub
 int bqb_ctx_download(bqb_ctx *c, int id, int node, void *host, size_t bytes);
 int bqb_ctx_upload(bqb_ctx *c, int id, int node, const void *host, size_t bytes);
 int bqb_ctx_sync(bqb_ctx *c);
+/* Heuristic plug point (src/heuristic.c:11-17 dispatches to
+ * runHeuristic_unpacked, src/biqbin_cpp_api.h:65, or to a host override,
+ * src/wrapper.cpp:210-226): a host function with runHeuristic_unpacked's
+ * arguments (plus a user pointer) replaces the device GW heuristic of the
+ * context at every point the reference calls runHeuristic; it receives
+ * SP->L, SP->n, the node's PP->L and PP->n, its xfixed and sol.X, and x (n0-1
+ * entries, fixed variables preset) to fill; the node's incumbent is raised
+ * with x as updateSolution does.  NULL restores the device heuristic. */
+typedef double (*bqb_heuristic_fn)(double *P0_L, int P0_N, double *P_L, int P_N, int *xfixed, int *solX,
+                                   int *x, void *user);
+int bqb_ctx_set_heuristic(bqb_ctx *c, bqb_heuristic_fn fn, void *user);
+
 /* Precision of a context's bounds (no counterpart in the reference; call
  * while no node is in flight): slices = 0 runs the IPM products in FP64 (the
  * reference's arithmetic), 2..4 on the bf16 matrix cores as in
@@ -364,7 +379,6 @@ int bqb_ctx_ipm_flops_mx(bqb_ctx *c, double *flops_mx);
  * W1[256][32] b1 W2[256][256] b2 W3[256][256] b3 W4[16][256] b4[16] (rows 2..15
  * of W4 and b4 unused, zero). */
 #define BQB_MLP_NPARAM 144144
-typedef struct bqb_mlp bqb_mlp;
 bqb_mlp *bqb_mlp_create(const float *params, uint64_t seed);   /* params NULL: random init */
 void bqb_mlp_destroy(bqb_mlp *m);
 int bqb_mlp_get_params(bqb_mlp *m, float *out);

@@ -255,7 +255,7 @@ def _declare_host(L):
     L.bqb_solve_cap.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, C.c_int,
                                 N.iptr, C.POINTER(SolveStats)]
     L.bqb_solve_ex.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, C.c_int,
-                               C.c_int, C.c_int, N.iptr, C.POINTER(SolveStats)]
+                               C.c_int, C.c_int, C.c_void_p, C.c_float, N.iptr, C.POINTER(SolveStats)]
     for f in ("Evaluate", "SDPbound"):
         getattr(L, f).argtypes = [C.POINTER(BabNode), C.POINTER(Problem), C.POINTER(Problem), C.c_int]
         getattr(L, f).restype = C.c_double
@@ -313,7 +313,7 @@ def evaluate_solution(SP_L, sol):
     return N.lib().bqb_evaluate_solution(SP_L.shape[0], _d(SP_L), _i(s))
 
 
-def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certify=0):
+def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certify=0, model=None, drop=0.0):
     """Branch and bound on the GPU (bqb_solve / bqb_solve_cap).  Returns
     (stats, solution) with the solution over the first n0-1 vertices (the
     last is on side 0)."""
@@ -322,8 +322,8 @@ def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certif
     p = params if params is not None else make_params()
     sol = np.zeros(n0 - 1, dtype=np.int32)
     st = SolveStats()
-    N.check(N.lib().bqb_solve_ex(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, slices, certify, _i(sol),
-                                 C.byref(st)))
+    N.check(N.lib().bqb_solve_ex(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, slices, certify,
+                                 model.h if model is not None else None, drop, _i(sol), C.byref(st)))
     return st, sol
 
 

@@ -272,11 +272,12 @@ static void count_flags(bqb_solve_stats *out, int flags)
 int bqb_solve_cap(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
                   int max_bundle, int *sol, bqb_solve_stats *out)
 {
-    return bqb_solve_ex(n0, SP_L, params, seed, batch, max_bundle, 0, 0, sol, out);
+    return bqb_solve_ex(n0, SP_L, params, seed, batch, max_bundle, 0, 0, NULL, 0.f, sol, out);
 }
 
 int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
-                 int max_bundle, int slices, int certify, int *sol, bqb_solve_stats *out)
+                 int max_bundle, int slices, int certify, const bqb_mlp *model, float drop, int *sol,
+                 bqb_solve_stats *out)
 {
     memset(out, 0, sizeof *out);
     const double t0 = now_s();
@@ -284,7 +285,10 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
     const int B = n0 - 1;
     bqb_ctx *ctx = bqb_ctx_create(n0, SP_L, params, batch, max_bundle, 0, 0, 0);
     if (!ctx) return -1;
-    if (bqb_ctx_set_precision(ctx, slices, certify)) { bqb_ctx_destroy(ctx); return -1; }
+    if (bqb_ctx_set_precision(ctx, slices, certify) || bqb_ctx_set_model(ctx, model, drop)) {
+        bqb_ctx_destroy(ctx);
+        return -1;
+    }
     qheap h = {0};
     int rc = -1;
     uint64_t next_id = 0;

@@ -40,6 +40,33 @@ extern RefTri *Cuts __attribute__((weak));
 extern RefPent *Pent_Cuts __attribute__((weak));
 extern RefHept *Hepta_Cuts __attribute__((weak));
 
+/* The host's heuristic, when it has one (src/heuristic.c:11-17): the
+ * override of the reference's Python build (src/wrapper.cpp:210-226) or a
+ * runHeuristic_unpacked of its own (src/biqbin_cpp_api.h:65).  Without
+ * either, the device GW heuristic runs. */
+extern double wrapped_heuristic(BqbProblem *P0, BqbProblem *P, BqbBabNode *node, int *x) __attribute__((weak));
+extern double runHeuristic_unpacked(double *P0_L, int P0_N, double *P_L, int P_N, int *node_xfixed,
+                                    int *node_sol_X, int *x) __attribute__((weak));
+
+static double call_wrapped(double *P0_L, int P0_N, double *P_L, int P_N, int *xfixed, int *solX, int *x,
+                           void *user)
+{
+    static BqbBabNode node;
+    (void)user;
+    BqbProblem P0 = {P0_L, P0_N, 0, 0, 0, 0}, P = {P_L, P_N, 0, 0, 0, 0};
+    memset(&node, 0, sizeof node);
+    memcpy(node.xfixed, xfixed, sizeof(int) * (P0_N - 1));
+    memcpy(node.solX, solX, sizeof(int) * (P0_N - 1));
+    return wrapped_heuristic(&P0, &P, &node, x);
+}
+
+static double call_unpacked(double *P0_L, int P0_N, double *P_L, int P_N, int *xfixed, int *solX, int *x,
+                            void *user)
+{
+    (void)user;
+    return runHeuristic_unpacked(P0_L, P0_N, P_L, P_N, xfixed, solX, x);
+}
+
 static struct {
     bqb_ctx *ctx;
     const double *SP_L;
@@ -73,6 +100,11 @@ static bqb_ctx *context_for(const BqbProblem *SP)
         g_drop.SP_L = SP->L;
         g_drop.n0 = SP->n;
         g_drop.p = p;
+        bqb_heuristic_fn h = wrapped_heuristic ? call_wrapped : runHeuristic_unpacked ? call_unpacked : NULL;
+        if (h && bqb_ctx_set_heuristic(g_drop.ctx, h, NULL)) {
+            fprintf(stderr, "Error: %s\n", bqb_last_error());
+            exit(10);
+        }
     }
     return g_drop.ctx;
 }

@@ -94,6 +94,13 @@ struct bqb_ctx {
     int *d_rec_key;            /* [rec_cap][12] */
     int *d_lidx;               /* [max_nodes][max_m] */
     unsigned char *d_lkeep;    /* [max_nodes][max_m] */
+    /* host heuristic replacing the device GW (bqb_ctx_set_heuristic) */
+    bqb_heuristic_fn heur_fn;
+    void *heur_user;
+    double *h_slab;            /* [ld*ld] one node's L slab */
+    double *h_PL;              /* [n0*n0] its n x n subproblem matrix */
+    int *h_mask, *h_xf, *h_sx, *h_x;
+    const double *h_SPL;       /* SP_L as given at creation (host copy) */
 };
 
 struct done_node {
@@ -116,6 +123,8 @@ static void traj_free(bqb_ctx *c)
 static void free_host(bqb_ctx *c)
 {
     traj_free(c);
+    free(c->h_slab); free(c->h_PL); free(c->h_mask); free(c->h_xf); free(c->h_sx); free(c->h_x);
+    free((void *)c->h_SPL);
     free(c->tr_f); free(c->tr_t); free(c->tr_bound); free(c->tr_cuts); free(c->tr_n);
     free(c->h_st); free(c->tag); free(c->free_stack); free(c->of_slot); free(c->ld_st);
     free(c->ld_xf); free(c->ld_sx); free(c->h_frac); free(c->h_xb);
@@ -274,6 +283,12 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         }
     }
     CK(hipMemcpy((void *)c->p.SP_L, SP_L, sizeof(double) * (size_t)n0 * n0, hipMemcpyHostToDevice), "H2D SP_L");
+    {
+        double *h = malloc(sizeof(double) * (size_t)n0 * n0);
+        if (!h) { bqb_set_error("calloc"); goto fail; }
+        memcpy(h, SP_L, sizeof(double) * (size_t)n0 * n0);
+        c->h_SPL = h;
+    }
     CK(hipMemcpy(c->p.params, &c->params, sizeof(BqbParams), hipMemcpyHostToDevice), "H2D params");
     CK(hipStreamCreate(&c->stream), "hipStreamCreate");
     CK(hipEventCreate(&c->ev0), "hipEventCreate");
@@ -365,6 +380,70 @@ static int ipm_launch(bqb_ctx *c, const double *L, double *X)
     return 0;
 }
 
+/* The heuristic phase with a host heuristic (src/heuristic.c:11-17 calls
+ * runHeuristic_unpacked or a host override): for every node of the phase
+ * mask, the host function gets the original objective SP->L, the node's
+ * subproblem matrix PP->L (as createSubproblem builds it), its fixings, and x
+ * (fixed variables preset, src/bounding.c:186-194); the node's incumbent is
+ * then raised as updateSolution does (src/bab_functions.c:114-134). */
+static int host_heuristic(bqb_ctx *c)
+{
+    const int B = c->d.n0 - 1, ld = c->d.ld, hi = c->d.nodes;
+    hipStream_t s = c->stream;
+    if (bqb_check(hipMemcpyAsync(c->h_mask, c->p.mask, sizeof(int) * hi, hipMemcpyDeviceToHost, s), "D2H mask") ||
+        bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * hi, hipMemcpyDeviceToHost, s), "D2H state") ||
+        bqb_check(hipMemcpyAsync(c->h_xf, c->p.xfixed, sizeof(int) * B * hi, hipMemcpyDeviceToHost, s), "D2H xfixed") ||
+        bqb_check(hipMemcpyAsync(c->h_sx, c->p.solX, sizeof(int) * B * hi, hipMemcpyDeviceToHost, s), "D2H solX") ||
+        bqb_check(hipStreamSynchronize(s), "sync"))
+        return -1;
+    int changed = 0;
+    for (int i = 0; i < hi; ++i) {
+        if (!c->h_mask[i]) continue;
+        NodeState *st = &c->h_st[i];
+        const int n = st->n;
+        if (bqb_check(hipMemcpy(c->h_slab, c->p.L + (size_t)i * ld * ld, sizeof(double) * ld * ld,
+                                hipMemcpyDeviceToHost), "D2H L"))
+            return -1;
+        for (int j = 0; j < n; ++j) memcpy(c->h_PL + (size_t)j * n, c->h_slab + (size_t)j * ld, sizeof(double) * n);
+        const int *xf = c->h_xf + (size_t)i * B, *sx = c->h_sx + (size_t)i * B;
+        for (int v = 0; v < B; ++v) c->h_x[v] = xf[v] ? sx[v] : 0;
+        c->heur_fn((double *)c->h_SPL, c->d.n0, c->h_PL, n, (int *)xf, (int *)sx, c->h_x, c->heur_user);
+        const double val = bqb_evaluate_solution(c->d.n0, c->h_SPL, c->h_x);
+        if (val > st->LB) {
+            st->LB = val;
+            st->improved = 1;
+            changed = 1;
+            if (bqb_check(hipMemcpy(c->p.xbest + (size_t)i * B, c->h_x, sizeof(int) * B, hipMemcpyHostToDevice),
+                          "H2D xbest"))
+                return -1;
+        }
+    }
+    if (changed && bqb_check(hipMemcpy(c->p.st, c->h_st, sizeof(NodeState) * hi, hipMemcpyHostToDevice), "H2D state"))
+        return -1;
+    return 0;
+}
+
+int bqb_ctx_set_heuristic(bqb_ctx *c, bqb_heuristic_fn fn, void *user)
+{
+    if (bqb_ctx_busy(c) > 0) { bqb_set_error("set_heuristic: nodes are in flight"); return -1; }
+    const size_t B = (size_t)(c->d.n0 - 1), N = (size_t)c->max_nodes, ld = (size_t)c->d.ld;
+    if (fn && !c->h_slab) {
+        c->h_slab = malloc(sizeof(double) * ld * ld);
+        c->h_PL = malloc(sizeof(double) * (size_t)c->d.n0 * c->d.n0);
+        c->h_mask = malloc(sizeof(int) * N);
+        c->h_xf = malloc(sizeof(int) * N * B);
+        c->h_sx = malloc(sizeof(int) * N * B);
+        c->h_x = malloc(sizeof(int) * (B + 1));
+        if (!c->h_slab || !c->h_PL || !c->h_mask || !c->h_xf || !c->h_sx || !c->h_x) {
+            bqb_set_error("calloc");
+            return -1;
+        }
+    }
+    c->heur_fn = fn;
+    c->heur_user = user;
+    return 0;
+}
+
 /* Run one phase kernel on the current batch (also used by tests). */
 int bqb_ctx_run(bqb_ctx *c, int what, int arg)
 {
@@ -374,7 +453,9 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     switch (what) {
     case 100: return ipm_launch(c, p->L, p->X);
     case 101: return ipm_launch(c, p->L0, p->Xt);
-    case 102: return bqb_check(bqb_launch_heur(d, p, c->heur_tile, c->spl_int, s), "heur launch");
+    case 102:
+        if (c->heur_fn) return host_heuristic(c);
+        return bqb_check(bqb_launch_heur(d, p, c->heur_tile, c->spl_int, s), "heur launch");
     case 103: return bqb_check(bqb_launch_trisep(d, p, c->params.TriIneq, s), "trisep launch");
     case 104: return bqb_check(bqb_launch_sa(d, p, 5, s), "sa5 launch");
     case 105: return bqb_check(bqb_launch_sa(d, p, 7, s), "sa7 launch");

@@ -14,6 +14,8 @@ LIB = os.path.join(ROOT, "parallel_biqbin_maxcut_amd", "libbiqbin_gpu.so")
 def declared_functions():
     txt = open(HDR).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    # function-pointer typedefs declare no symbol
+    txt = re.sub(r"^typedef[^;]*;", "", txt, flags=re.M | re.S)
     names = re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", txt, flags=re.M)
     return sorted(set(names))
 

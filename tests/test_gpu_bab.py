@@ -120,3 +120,30 @@ def test_dropin_evaluate_root_node(golden):
     assert bound == r["bound"][0]
     assert lb == r["LB"][0] and lb == api.evaluate_solution(SP, sol)
     assert np.array_equal(np.array(node.fracsol[: n0 - 1]), r["fracsol"][0])
+
+
+@pytest.mark.parametrize("name", ["g05_60.0", "g05_80.0"])
+def test_fixture_rudy_json_copies(golden, name):
+    """The reference's adjacency-JSON copies of the rudy fixtures (read as
+    biqbin_base.py:33-45, 76-89 does) prove the same optimum."""
+    path = os.path.join(golden, "rudy", name)
+    adj = I.read_sparse_json(path + ".json", "adjacency")
+    SP = api.maxcut_L(adj)
+    st, sol = api.solve(SP, reference_params(SP.shape[0]), seed=2020, batch=32)
+    assert st.status == 0
+    assert st.best == fixture_value(path + "-expected_output")
+
+
+def test_fixture_kcluster80_qubo(golden):
+    """kcluster80 (QUBO on 80 variables -> Max-Cut on 81 vertices): the
+    fixture's optimum 16400 / QUBO -16400 and a solution attaining it."""
+    base = os.path.join(golden, "qubos", "kcluster80_025_20_1.json")
+    Q = I.read_sparse_json(base, "qubo")
+    exp = json.load(open(base + "-expected_output.json"))
+    SP = api.maxcut_L(I.qubo_to_maxcut_adj(Q))
+    st, sol = api.solve(SP, reference_params(SP.shape[0]), seed=2020, batch=64)
+    print(f"kcluster80: {st.nodes} nodes, {st.seconds:.2f} s", flush=True)
+    assert st.status == 0
+    assert st.best == pytest.approx(exp["maxcut"]["computed_val"], abs=1e-6)
+    y = I.maxcut_solution_to_qubo([i + 1 for i in range(len(sol)) if sol[i]], Q.shape[0])
+    assert y @ Q @ y == pytest.approx(exp["qubo"]["computed_val"], abs=1e-6)

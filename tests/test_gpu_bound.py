@@ -495,3 +495,22 @@ def test_trajectory_capture_records_rounds_cuts_and_gamma():
     ctx.bound(xf[:3], sx[:3], LB=0.0, diff=1e9, seeds=seeds[:3], stream=True)
     assert ctx.trajectories() == []
     ctx.close()
+
+
+def test_cfg4_qubo_nodes_three_rounds_match_oracle():
+    """QUBO n = 150 with entries in {-100..100} at 50 % density as Max-Cut on
+    151 vertices: four nodes with 0-40 fixings (sizes 111-151), three outer
+    rounds with triangle cuts: same decisions and bounds within 1e-6 of the
+    oracle (heuristic, separation and IPM on QUBO-scale weights)."""
+    n0 = 151
+    SP = O.build_L(I.qubo_to_maxcut_adj(I.random_qubo(150, 0.5, 2026)))
+    P = api.make_params(include_Pent=0, include_Hepta=0, max_outer_iter=3, extra_iter=0)
+    xf, sx = I.random_fixings(n0, 40, 4, 2027)
+    r, outs = _bound_pair(SP, P, xf, sx, 0.0, 0.0, 0, np.arange(1, 5, dtype=np.uint64))
+    for b in range(4):
+        o = outs[b][0]
+        assert r["status"][b] == 0 and o.status == 0
+        s = r["stats"][b]
+        assert (s[0], s[1], s[2], s[5]) == (o.prune, o.giveup, o.outer_iters, o.max_cuts), b
+        assert abs(r["bound"][b] - o.bound) <= 1e-6 * abs(o.bound), (b, r["bound"][b], o.bound)
+        assert r["LB"][b] == o.LB

@@ -73,3 +73,15 @@ def test_evaluate_solution_and_brute_force_value():
         assert v == O.evaluate_solution(SP, x)
         best = max(best, v)
     assert best == pytest.approx(brute_maxcut(adj))
+
+
+def test_rudy_json_copies_match_rudy_files(golden):
+    """The reference's adjacency-JSON copies hold the same graphs as the
+    rudy files (both triangles stored, integer weights)."""
+    import os
+    from parallel_biqbin_maxcut_amd import instances as I
+    for name in ("g05_60.0", "g05_80.0", "g05_100.4"):
+        path = os.path.join(golden, "rudy", name)
+        a = I.read_rudy(path)
+        b = I.read_sparse_json(path + ".json", "adjacency")
+        assert np.array_equal(a, b), name

@@ -257,6 +257,51 @@ def traffic_per_launch():
         return None
 
 
+def subproblem_matrices(api, SP, xf, sx):
+    """The subproblem matrices of the given fixings, built by the library's
+    drop-in createSubproblem (src/evaluate.c:38-115)."""
+    n0 = SP.shape[0]
+    buf = np.zeros(n0 * n0)
+    prob = api.Problem(SP.ctypes.data_as(api.N.dptr), n0, 0, 0, 0, 0)
+    sub = api.Problem(buf.ctypes.data_as(api.N.dptr), n0, 0, 0, 0, 0)
+    out = []
+    for b in range(len(xf)):
+        node = api.BabNode()
+        node.xfixed[: n0 - 1] = list(xf[b])
+        node.solX[: n0 - 1] = list(sx[b])
+        api.N.lib().createSubproblem(node, prob, sub)
+        out.append(buf[: sub.n * sub.n].reshape((sub.n, sub.n), order="F").copy())
+    return out
+
+
+def ipm_isolated(api, I, seed, nodes, reps):
+    """The batched IPM alone on one stream: the basic SDP of `nodes`
+    subproblems of the batched-bounding instance (sizes 60-100), one launch
+    per repetition, timed by HIP events around the launch."""
+    SP, _, xf, sx, _ = build_workload(api, I, seed, nodes)
+    Ls = subproblem_matrices(api, SP, xf, sx)
+    api.ipm_batch_mixed(Ls, slices=0)                   # warm-up
+    ms = []
+    for _ in range(reps):
+        r = api.ipm_batch_mixed(Ls, slices=0)
+        if np.any(r["status"]):
+            raise SystemExit(f"IPM failures: {np.unique(r['status'], return_counts=True)}")
+        ms.append(r["ms"])
+    ns = np.array([m.shape[0] for m in Ls], dtype=np.float64)
+    flops = float(np.sum(ns ** 3 * (6.0 * r["iters"] + r["trials"] / 3.0)))
+    t = float(np.median(ms))
+    achieved = flops / (t / 1e3) / 1e12
+    # algorithmic HBM bytes: L read and X written once per subproblem
+    alg_bytes = float(np.sum(2 * 8 * ns ** 2))
+    return {"mode": "ipm_isolated", "kernel": "ipm_batch_kernel", "workload":
+            f"basic SDP of {nodes} subproblems of Max-Cut G(100, 0.3) +-1 with 0-40 fixings (sizes "
+            f"{int(ns.min())}-{int(ns.max())}), one launch on one stream", "launch_ms": round(t, 4),
+            "launch_ms_all": [round(x, 4) for x in ms], "gflop_per_launch": round(flops / 1e9, 4),
+            "mean_iters": round(float(r["iters"].mean()), 3), "mean_trials": round(float(r["trials"].mean()), 3),
+            "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "algorithmic_hbm_bytes": alg_bytes}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,6 +318,10 @@ def main():
                     help="skip the cfg3 mixed-precision and cfg4 QUBO sections")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-batch comparison after the timed steps (profiling runs)")
+    ap.add_argument("--graphs", type=int, default=0,
+                    help="1: launch trains as HIP graphs (basic sequence and outer-round slices)")
+    ap.add_argument("--ipm-isolated", action="store_true",
+                    help="time the batched IPM alone on --nodes subproblems, one launch per step (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: bounding runs on one GPU; --gpus {args.gpus} is not supported "
@@ -281,7 +330,12 @@ def main():
 
     from parallel_biqbin_maxcut_amd import api, instances as I
 
+    if args.ipm_isolated:
+        print(json.dumps(ipm_isolated(api, I, args.seed, args.nodes, max(1, args.steps))), flush=True)
+        return
+
     SP, P, xf, sx, seeds = build_workload(api, I, args.seed, args.per_step)
+    api.set_default_graphs(bool(args.graphs))
     ctx = api.BoundContext(SP, params=P, max_nodes=args.nodes, groups=args.groups)
     # root bound of the instance: incumbent and root gap the workers receive
     z = np.zeros((1, N0 - 1), np.int32)
@@ -292,14 +346,27 @@ def main():
 
     for _ in range(args.warmup):
         ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
-    ctx.profile_ipm(max_launches=5000 * max(1, args.steps))
+    # the IPM launch profile (event pairs around each IPM launch) keeps the
+    # trains eager; with graphs the flops are counted in an untimed pass
+    if not args.graphs:
+        ctx.profile_ipm(max_launches=5000 * max(1, args.steps))
     ctx.sync()
+    ctx.counters(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
     ctx.sync()
     dt = time.perf_counter() - t0
+    counters = ctx.counters()
+    if args.graphs:
+        ctx.set_graphs(False)
+        ctx.profile_ipm(max_launches=5000)
+        r2 = ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
+        if not np.array_equal(r2["bound"], r["bound"]):
+            raise SystemExit("graph and eager launch trains gave different bounds")
     launches, ipm_ms, flops = ctx.ipm_stats()
+    if args.graphs:
+        flops *= args.steps
     ctx.profile_ipm(0)
     if np.any(r["status"]):
         raise SystemExit(f"node bound failures: {np.unique(r['status'], return_counts=True)}")
@@ -352,6 +419,13 @@ def main():
             "pruned_after_basic_bound": int(np.sum((stats[:, 0] == 1) & (stats[:, 2] == 0))),
             "mean_outer_rounds": round(float(stats[:, 2].mean()), 3),
             "ipm_calls": int(stats[:, 3].sum()),
+            "graphs": bool(args.graphs),
+            "launch_counters_per_step": {k: round(v / args.steps, 1) for k, v in counters.items()},
+            "host_syncs_per_round_slice": round(counters["host_syncs"] / max(1, counters["round_slices"]), 3),
+            "kernel_phases_per_round_slice": round((counters["eager_phases"] + counters["graph_phases"])
+                                                   / max(1, counters["round_slices"]), 3),
+            "host_launch_calls_per_round_slice": round((counters["eager_phases"] + counters["graph_launches"])
+                                                       / max(1, counters["round_slices"]), 3),
         },
         "roofline": {
             "kernel": "ipm_batch_kernel",

@@ -415,6 +415,25 @@ int bqb_ctx_learn_records(bqb_ctx *c, int *count, float *feat, int *key);
 int bqb_ctx_profile_ipm(bqb_ctx *c, int max_launches);
 int bqb_ctx_ipm_stats(bqb_ctx *c, int *launches, double *ms_total, double *flops);
 
+/* Launch trains as HIP graphs (no counterpart in the reference, whose loop
+ * runs on the host, src/bounding.c:167-330).  on = 1: the basic sequence of
+ * newly loaded nodes and each outer-round slice (bundle iterations, heuristic,
+ * separation, rebuild) are captured once per shape (occupied slots, bundle
+ * iterations in the slice, bundle-size class) and replayed with one
+ * hipGraphLaunch; results are bitwise those of the eager launches.  Graphs
+ * are not used while the IPM profile is on or a host heuristic is set.
+ * Off by default. */
+int bqb_ctx_set_graphs(bqb_ctx *c, int on);
+/* The graph setting of contexts created afterwards (bqb_solve included);
+ * returns the previous one. */
+int bqb_set_default_graphs(int on);
+/* Host-side counters since creation (or the last reset with reset = 1):
+ * out[0] advance calls, out[1] outer-round slices, out[2] phases launched
+ * eagerly, out[3] graph launches, out[4] phases replayed inside graphs,
+ * out[5] blocking host synchronisations, out[6] graph instantiations. */
+#define BQB_NCOUNTERS 7
+int bqb_ctx_counters(bqb_ctx *c, long long *out, int reset);
+
 /* fct_eval (src/bundle.c:242-275) on explicit cut lists, using node slot 0
  * of a context whose tile fits n: X = IPM solution for L - B'(gamma)
  * (n x n column-major in and out; X may be NULL), g = 1 - B(X) (m entries,

@@ -313,17 +313,24 @@ def evaluate_solution(SP_L, sol):
     return N.lib().bqb_evaluate_solution(SP_L.shape[0], _d(SP_L), _i(s))
 
 
-def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certify=0, model=None, drop=0.0):
+def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certify=0, model=None, drop=0.0,
+          graphs=None):
     """Branch and bound on the GPU (bqb_solve / bqb_solve_cap).  Returns
     (stats, solution) with the solution over the first n0-1 vertices (the
-    last is on side 0)."""
+    last is on side 0).  graphs: launch trains as HIP graphs (None: the
+    library's default)."""
     SP_L = np.asfortranarray(SP_L, dtype=np.float64)
     n0 = SP_L.shape[0]
     p = params if params is not None else make_params()
     sol = np.zeros(n0 - 1, dtype=np.int32)
     st = SolveStats()
-    N.check(N.lib().bqb_solve_ex(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, slices, certify,
-                                 model.h if model is not None else None, drop, _i(sol), C.byref(st)))
+    prev = set_default_graphs(graphs) if graphs is not None else None
+    try:
+        N.check(N.lib().bqb_solve_ex(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, slices, certify,
+                                     model.h if model is not None else None, drop, _i(sol), C.byref(st)))
+    finally:
+        if prev is not None:
+            set_default_graphs(prev)
     return st, sol
 
 
@@ -367,6 +374,20 @@ def _declare_ctx(L):
     L.bqb_ctx_ipm_stats.argtypes = [vp, N.iptr, N.dptr, N.dptr]
     L.bqb_ctx_set_precision.argtypes = [vp, C.c_int, C.c_int]
     L.bqb_ctx_ipm_flops_mx.argtypes = [vp, N.dptr]
+    L.bqb_ctx_set_graphs.argtypes = [vp, C.c_int]
+    L.bqb_set_default_graphs.argtypes = [C.c_int]
+    L.bqb_ctx_counters.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
+
+
+NCOUNTERS = 7
+COUNTER_NAMES = ("advances", "round_slices", "eager_phases", "graph_launches", "graph_phases", "host_syncs",
+                 "graph_instantiations")
+
+
+def set_default_graphs(on):
+    """Launch trains as HIP graphs in contexts created afterwards (bqb_solve
+    included); returns the previous setting."""
+    return N.lib().bqb_set_default_graphs(1 if on else 0)
 
 
 class BoundContext:
@@ -545,6 +566,20 @@ class BoundContext:
             N.check(N.lib().bqb_ctx_ipm_flops_mx(h, C.byref(f)))
             tot += f.value
         return tot
+
+    def set_graphs(self, on=True):
+        for h in self.handles:
+            N.check(N.lib().bqb_ctx_set_graphs(h, 1 if on else 0))
+
+    def counters(self, reset=False):
+        """Host-side launch and synchronisation counters summed over the slot
+        groups (dict keyed by COUNTER_NAMES)."""
+        tot = np.zeros(NCOUNTERS, dtype=np.int64)
+        buf = (C.c_longlong * NCOUNTERS)()
+        for h in self.handles:
+            N.check(N.lib().bqb_ctx_counters(h, buf, 1 if reset else 0))
+            tot += np.array(buf[:], dtype=np.int64)
+        return dict(zip(COUNTER_NAMES, (int(v) for v in tot)))
 
     def set_precision(self, slices=0, certify=0):
         """IPM products in FP64 (slices 0) or on the bf16 matrix cores

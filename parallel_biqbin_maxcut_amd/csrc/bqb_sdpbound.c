@@ -41,6 +41,18 @@ hipError_t bqb_launch_learn(const BatchDims *d, const BatchPtrs *p, const bqb_ml
                             int rec_cap, float *rec_feat, int *rec_key, int *idx, unsigned char *keep,
                             int after_round, hipStream_t s);
 
+/* A captured launch train: kind 0 the basic sequence of newly loaded nodes
+ * (key: occupied slots), kind 1 an outer-round slice (key: occupied slots,
+ * bundle iterations, largest bundle clamped to 17, the last size where the
+ * QP variant choice still changes). */
+#define BQB_GRAPH_CACHE 16
+struct train_graph {
+    int kind, hi, nb, kcls;
+    int phases;
+    unsigned long long stamp;  /* 0: empty */
+    hipGraphExec_t exec;
+};
+
 struct bqb_ctx {
     BatchDims d;
     BatchPtrs p;
@@ -101,6 +113,13 @@ struct bqb_ctx {
     double *h_PL;              /* [n0*n0] its n x n subproblem matrix */
     int *h_mask, *h_xf, *h_sx, *h_x;
     const double *h_SPL;       /* SP_L as given at creation (host copy) */
+    /* launch trains captured as HIP graphs, keyed by their shape */
+    int graphs;
+    struct train_graph gcache[BQB_GRAPH_CACHE];
+    unsigned long long gstamp;
+    int capturing;             /* phases go into a capture, not counted as eager */
+    int cap_phases;
+    long long cnt[BQB_NCOUNTERS];
 };
 
 struct done_node {
@@ -111,6 +130,25 @@ struct done_node {
 };
 
 #define CK(call, what) do { if (bqb_check((call), (what))) goto fail; } while (0)
+
+/* graph setting of new contexts (bqb_set_default_graphs) */
+static int g_default_graphs = 0;
+
+/* A blocking wait for the context's stream (counted). */
+static int ctx_sync(bqb_ctx *c)
+{
+    c->cnt[5]++;
+    return bqb_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+static void graphs_clear(bqb_ctx *c)
+{
+    for (int i = 0; i < BQB_GRAPH_CACHE; ++i)
+        if (c->gcache[i].stamp) {
+            hipGraphExecDestroy(c->gcache[i].exec);
+            c->gcache[i].stamp = 0;
+        }
+}
 
 static void traj_free(bqb_ctx *c)
 {
@@ -308,6 +346,7 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         goto fail;
     }
     engine_reset(c);
+    c->graphs = g_default_graphs;
     return c;
 fail:
     if (c->arena) hipFree(c->arena);
@@ -328,6 +367,7 @@ void bqb_ctx_destroy(bqb_ctx *c)
 {
     if (!c) return;
     hipStreamSynchronize(c->stream);
+    graphs_clear(c);
     if (c->d_rec_count) hipFree(c->d_rec_count);
     if (c->d_lidx) hipFree(c->d_lidx);
     free_events(c);
@@ -394,7 +434,7 @@ static int host_heuristic(bqb_ctx *c)
         bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * hi, hipMemcpyDeviceToHost, s), "D2H state") ||
         bqb_check(hipMemcpyAsync(c->h_xf, c->p.xfixed, sizeof(int) * B * hi, hipMemcpyDeviceToHost, s), "D2H xfixed") ||
         bqb_check(hipMemcpyAsync(c->h_sx, c->p.solX, sizeof(int) * B * hi, hipMemcpyDeviceToHost, s), "D2H solX") ||
-        bqb_check(hipStreamSynchronize(s), "sync"))
+        ctx_sync(c))
         return -1;
     int changed = 0;
     for (int i = 0; i < hi; ++i) {
@@ -450,6 +490,8 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     const BatchDims *d = &c->d;
     const BatchPtrs *p = &c->p;
     hipStream_t s = c->stream;
+    if (c->capturing) c->cap_phases++;
+    else c->cnt[2]++;
     switch (what) {
     case 100: return ipm_launch(c, p->L, p->X);
     case 101: return ipm_launch(c, p->L0, p->Xt);
@@ -535,14 +577,14 @@ static int upload_staged(bqb_ctx *c)
     for (int i = 0; i < c->hi; ++i) c->of_slot[i] = -1;
     c->nload = 0;
     /* the host copies of the staging must stay valid until the copies ran */
-    return bqb_check(hipStreamSynchronize(s), "sync");
+    return ctx_sync(c);
 }
 
 static int fetch_states(bqb_ctx *c)
 {
     if (bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * c->hi, hipMemcpyDeviceToHost, c->stream), "D2H state"))
         return -1;
-    return bqb_check(hipStreamSynchronize(c->stream), "sync");
+    return ctx_sync(c);
 }
 
 static int done_push(bqb_ctx *c, long long tag, const NodeState *st, const double *frac, const int *xb)
@@ -657,7 +699,7 @@ static int collect(bqb_ctx *c)
     if (c->capture)
         for (int i = 0; i < c->hi; ++i)
             if (c->tag[i] >= 0 && !c->h_st[i].active && traj_start(c, i)) return -1;
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     for (int i = 0; i < c->hi; ++i) {
         if (c->tag[i] < 0 || c->h_st[i].active) continue;
         if (done_push(c, c->tag[i], &c->h_st[i], c->h_frac + i * B, c->h_xb + i * B)) return -1;
@@ -673,9 +715,135 @@ static int collect(bqb_ctx *c)
     return 0;
 }
 
+/* The basic sequence of newly loaded nodes (bounding.c:24-163). */
+static int basic_train(bqb_ctx *c)
+{
+    RUN(PH_SETUP, 0);
+    RUN(100, 0);             /* basic IPM: L -> X */
+    RUN(PH_POST_BASIC, 0);
+    if (c->d.certify) RUN(110, 0);   /* certified basic bound */
+    RUN(102, 0);             /* heuristic */
+    RUN(PH_CHECK_BASIC, 0);
+    RUN(103, 0);             /* first triangle separation */
+    RUN(112, 0);             /* learned warm start / ranking of the new cuts */
+    RUN(PH_BUNDLE_INIT, 0);
+    RUN(106, 0);             /* op_Bt gather lists */
+    return 0;
+}
+
+/* One outer-round slice: nb bundle iterations, then the round end
+ * (bounding.c:170-326).  kmax bounds the bundle sizes at the slice start. */
+static int round_train(bqb_ctx *c, int nb, int kmax)
+{
+    RUN(PH_OUTER_BEGIN, 0);
+    for (int b = 0; b < nb; ++b) {
+        RUN(PH_BUNDLE_NEXT, b);
+        /* QP -> gamma_test; each bundle iteration appends at most one
+         * element, so no bundle exceeds kmax + b (the QP kernel sizes its
+         * LDS from this bound) */
+        RUN(108, kmax + b);
+        RUN(107, 1);         /* L0 = L - B'(gamma_test) */
+        RUN(101, 0);         /* IPM: L0 -> X_test */
+        RUN(109, 0);         /* serious / null step */
+        if (c->d.certify) RUN(111, 0);   /* certify the new center */
+    }
+    RUN(PH_OUTER_A, 0);
+    RUN(102, 0);             /* heuristic */
+    RUN(PH_OUTER_B, 0);
+    RUN(103, 0);             /* triangles */
+    RUN(PH_SEP_PENT, 0);
+    RUN(104, 0);             /* pentagonal */
+    RUN(PH_SEP_HEPT, 0);
+    RUN(105, 0);             /* heptagonal */
+    RUN(113, 0);             /* learned warm start / ranking of the new cuts */
+    RUN(PH_OUTER_C, 0);
+    RUN(106, 0);             /* new gather lists */
+    RUN(107, 0);             /* L0 = L - B'(gamma) */
+    RUN(101, 0);             /* IPM: L0 -> X_test */
+    RUN(PH_REBUILD, 0);
+    if (c->d.learned && c->d.certify) RUN(114, 0);   /* certify the warm-started center */
+    return 0;
+}
+
+/* Runs a train through its captured graph (capturing it on first use):
+ * kind 0 basic_train, kind 1 round_train(nb, kmax).  The kernels and their
+ * arguments are exactly the eager launches', in the same order on the same
+ * stream. */
+static int train_graph(bqb_ctx *c, int kind, int nb, int kmax)
+{
+    /* the QP launch depends on kmax + b only through its class (<= 0,
+     * <= 16, > 16): above 16 every kmax gives the same train */
+    const int kcls = kind ? (kmax > 17 ? 17 : kmax) : 0;
+    if (!kind) nb = 0;
+    struct train_graph *g = NULL, *victim = &c->gcache[0];
+    for (int i = 0; i < BQB_GRAPH_CACHE; ++i) {
+        struct train_graph *e = &c->gcache[i];
+        if (e->stamp && e->kind == kind && e->hi == c->hi && e->nb == nb && e->kcls == kcls) { g = e; break; }
+        if (e->stamp < victim->stamp) victim = e;
+    }
+    if (!g) {
+        if (victim->stamp) {
+            hipGraphExecDestroy(victim->exec);
+            victim->stamp = 0;
+        }
+        hipGraph_t graph = NULL;
+        if (bqb_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture")) return -1;
+        c->capturing = 1;
+        c->cap_phases = 0;
+        const int rc = kind ? round_train(c, nb, kmax) : basic_train(c);
+        c->capturing = 0;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc || bqb_check(e, "end capture")) {
+            if (graph) hipGraphDestroy(graph);
+            return -1;
+        }
+        const hipError_t ei = hipGraphInstantiate(&victim->exec, graph, NULL, NULL, 0);
+        hipGraphDestroy(graph);
+        if (bqb_check(ei, "graph instantiate")) return -1;
+        victim->kind = kind;
+        victim->hi = c->hi;
+        victim->nb = nb;
+        victim->kcls = kcls;
+        victim->phases = c->cap_phases;
+        c->cnt[6]++;
+        g = victim;
+    }
+    g->stamp = ++c->gstamp;
+    c->cnt[3]++;
+    c->cnt[4] += g->phases;
+    return bqb_check(hipGraphLaunch(g->exec, c->stream), "graph launch");
+}
+
+/* Graphs replace the eager trains unless a host step or an event-timed
+ * launch sits inside them. */
+static int use_graphs(const bqb_ctx *c) { return c->graphs && !c->heur_fn && !c->profile_ipm; }
+
+int bqb_set_default_graphs(int on)
+{
+    const int prev = g_default_graphs;
+    g_default_graphs = on ? 1 : 0;
+    return prev;
+}
+
+int bqb_ctx_set_graphs(bqb_ctx *c, int on)
+{
+    if (!on) graphs_clear(c);
+    c->graphs = on ? 1 : 0;
+    return 0;
+}
+
+int bqb_ctx_counters(bqb_ctx *c, long long *out, int reset)
+{
+    if (out)
+        for (int i = 0; i < BQB_NCOUNTERS; ++i) out[i] = c->cnt[i];
+    if (reset) memset(c->cnt, 0, sizeof c->cnt);
+    return 0;
+}
+
 int bqb_ctx_advance(bqb_ctx *c)
 {
     const BqbParams *P = &c->params;
+    c->cnt[0]++;
     if (c->nload) {
         if (upload_staged(c)) return -1;
         c->loaded = 1;
@@ -683,16 +851,7 @@ int bqb_ctx_advance(bqb_ctx *c)
     if (c->hi == 0) return 0;
     c->d.nodes = c->hi;
     if (c->loaded) {
-        RUN(PH_SETUP, 0);
-        RUN(100, 0);             /* basic IPM: L -> X */
-        RUN(PH_POST_BASIC, 0);
-        if (c->d.certify) RUN(110, 0);   /* certified basic bound */
-        RUN(102, 0);             /* heuristic */
-        RUN(PH_CHECK_BASIC, 0);
-        RUN(103, 0);             /* first triangle separation */
-        RUN(112, 0);             /* learned warm start / ranking of the new cuts */
-        RUN(PH_BUNDLE_INIT, 0);
-        RUN(106, 0);             /* op_Bt gather lists */
+        if (use_graphs(c) ? train_graph(c, 0, 0, 0) : basic_train(c)) return -1;
         c->loaded = 0;
     }
     if (fetch_states(c)) return -1;
@@ -720,33 +879,8 @@ int bqb_ctx_advance(bqb_ctx *c)
     }
     if (in_loop) {
         const int nb = bmax < slice ? bmax : slice;
-        RUN(PH_OUTER_BEGIN, 0);
-        for (int b = 0; b < nb; ++b) {
-            RUN(PH_BUNDLE_NEXT, b);
-            /* QP -> gamma_test; each bundle iteration appends at most one
-             * element, so no bundle exceeds kmax + b (the QP kernel sizes its
-             * LDS from this bound) */
-            RUN(108, kmax + b);
-            RUN(107, 1);         /* L0 = L - B'(gamma_test) */
-            RUN(101, 0);         /* IPM: L0 -> X_test */
-            RUN(109, 0);         /* serious / null step */
-            if (c->d.certify) RUN(111, 0);   /* certify the new center */
-        }
-        RUN(PH_OUTER_A, 0);
-        RUN(102, 0);             /* heuristic */
-        RUN(PH_OUTER_B, 0);
-        RUN(103, 0);             /* triangles */
-        RUN(PH_SEP_PENT, 0);
-        RUN(104, 0);             /* pentagonal */
-        RUN(PH_SEP_HEPT, 0);
-        RUN(105, 0);             /* heptagonal */
-        RUN(113, 0);             /* learned warm start / ranking of the new cuts */
-        RUN(PH_OUTER_C, 0);
-        RUN(106, 0);             /* new gather lists */
-        RUN(107, 0);             /* L0 = L - B'(gamma) */
-        RUN(101, 0);             /* IPM: L0 -> X_test */
-        RUN(PH_REBUILD, 0);
-        if (c->d.learned && c->d.certify) RUN(114, 0);   /* certify the warm-started center */
+        c->cnt[1]++;
+        if (use_graphs(c) ? train_graph(c, 1, nb, kmax) : round_train(c, nb, kmax)) return -1;
         if (fetch_states(c)) return -1;
     }
     if (c->capture) log_rounds(c);
@@ -849,7 +983,7 @@ int bqb_ctx_results(bqb_ctx *c, double *bound, double *fracsol, int *xbest, doub
     if (bqb_check(hipMemcpyAsync(c->h_st, c->p.st, sizeof(NodeState) * nodes, hipMemcpyDeviceToHost, c->stream), "D2H state")) return -1;
     if (fracsol && bqb_check(hipMemcpyAsync(fracsol, c->p.fracsol, sizeof(double) * nodes * B, hipMemcpyDeviceToHost, c->stream), "D2H fracsol")) return -1;
     if (xbest && bqb_check(hipMemcpyAsync(xbest, c->p.xbest, sizeof(int) * nodes * B, hipMemcpyDeviceToHost, c->stream), "D2H xbest")) return -1;
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     for (int i = 0; i < nodes; ++i) {
         const NodeState *s = &c->h_st[i];
         if (bound) bound[i] = s->bound;
@@ -1117,7 +1251,7 @@ int bqb_fct_eval(bqb_ctx *c, int n, const double *L, int ntri, const int *tri, i
     if (bqb_check(hipMemcpyAsync(&st, c->p.st, sizeof st, hipMemcpyDeviceToHost, s), "D2H state")) goto out;
     if (X && bqb_check(hipMemcpyAsync(hL, c->p.Xt, sizeof(double) * slab, hipMemcpyDeviceToHost, s), "D2H X")) goto out;
     if (g && m && bqb_check(hipMemcpyAsync(g, c->p.g, sizeof(double) * m, hipMemcpyDeviceToHost, s), "D2H g")) goto out;
-    if (bqb_check(hipStreamSynchronize(s), "sync")) goto out;
+    if (ctx_sync(c)) goto out;
     if (X)
         for (int j = 0; j < n; ++j) memcpy(X + (size_t)j * n, hL + (size_t)j * ld, sizeof(double) * n);
     if (f) *f = st.f_test;
@@ -1176,7 +1310,7 @@ int bqb_ctx_download(bqb_ctx *c, int id, int node, void *host, size_t bytes)
     char *ptr; size_t cap;
     if (node < 0 || node >= c->max_nodes || buffer_of(c, id, node, &ptr, &cap)) return -1;
     if (bytes > cap) bytes = cap;
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     return bqb_check(hipMemcpy(host, ptr, bytes, hipMemcpyDeviceToHost), "D2H buffer");
 }
 
@@ -1185,18 +1319,18 @@ int bqb_ctx_upload(bqb_ctx *c, int id, int node, const void *host, size_t bytes)
     char *ptr; size_t cap;
     if (node < 0 || node >= c->max_nodes || buffer_of(c, id, node, &ptr, &cap)) return -1;
     if (bytes > cap) { bqb_set_error("upload larger than buffer"); return -1; }
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     return bqb_check(hipMemcpy(ptr, host, bytes, hipMemcpyHostToDevice), "H2D buffer");
 }
 
-int bqb_ctx_sync(bqb_ctx *c) { return bqb_check(hipStreamSynchronize(c->stream), "sync"); }
+int bqb_ctx_sync(bqb_ctx *c) { return ctx_sync(c); }
 
 /* Start (max_launches > 0) or stop (0) timing every IPM launch with an event
  * pair and counting its algorithmic flops on the device.  Starting resets
  * the counters. */
 int bqb_ctx_profile_ipm(bqb_ctx *c, int max_launches)
 {
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     if (max_launches <= 0) { c->profile_ipm = 0; return 0; }
     if (2 * max_launches > c->pev_cap) {
         free_events(c);
@@ -1230,12 +1364,14 @@ int bqb_ctx_set_model(bqb_ctx *c, const bqb_mlp *model, float drop)
     c->model = model;
     c->drop = model ? drop : 0.f;
     c->d.learned = (model && !c->rec_cap) ? 1 : 0;
+    graphs_clear(c);
     return 0;
 }
 
 int bqb_ctx_learn_record(bqb_ctx *c, int capacity)
 {
     if (bqb_ctx_busy(c) > 0) { bqb_set_error("learn_record: nodes are in flight"); return -1; }
+    graphs_clear(c);
     if (c->d_rec_count) { hipFree(c->d_rec_count); c->d_rec_count = NULL; }
     c->rec_cap = 0;
     if (capacity <= 0) return 0;
@@ -1253,7 +1389,7 @@ int bqb_ctx_learn_records(bqb_ctx *c, int *count, float *feat, int *key)
 {
     int k = 0;
     if (!c->rec_cap) { *count = 0; return 0; }
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     if (bqb_check(hipMemcpy(&k, c->d_rec_count, sizeof(int), hipMemcpyDeviceToHost), "D2H count")) return -1;
     if (k > c->rec_cap) k = c->rec_cap;
     *count = k;
@@ -1274,13 +1410,14 @@ int bqb_ctx_set_precision(bqb_ctx *c, int slices, int certify)
     c->d.mix_slices = slices;
     c->d.certify = (certify || slices > 0) ? 1 : 0;
     c->d.work_slabs = BQB_IPM_WORK_SLABS(slices);
+    graphs_clear(c);
     return 0;
 }
 
 int bqb_ctx_ipm_flops_mx(bqb_ctx *c, double *flops_mx)
 {
     unsigned long long f[2] = {0, 0};
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     if (bqb_check(hipMemcpy(f, c->d_flops, sizeof f, hipMemcpyDeviceToHost), "D2H flops")) return -1;
     *flops_mx = (double)f[1];
     return 0;
@@ -1288,7 +1425,7 @@ int bqb_ctx_ipm_flops_mx(bqb_ctx *c, double *flops_mx)
 
 int bqb_ctx_ipm_stats(bqb_ctx *c, int *launches, double *ms_total, double *flops)
 {
-    if (bqb_check(hipStreamSynchronize(c->stream), "sync")) return -1;
+    if (ctx_sync(c)) return -1;
     double ms = 0.0;
     for (int i = 0; i + 1 < c->pev_n; i += 2) {
         float e = 0.f;

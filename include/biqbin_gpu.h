@@ -78,6 +78,9 @@ typedef struct BiqBinParameters {
 /* An interior-point solve of the node with products on the bf16 matrix
  * cores failed and was repeated with FP64 products. */
 #define BQB_FLAG_FP64_RETRY 16
+/* An interior-point solve of the node by the blocked kernel (large
+ * subproblems) failed and was repeated with the register-tile kernel. */
+#define BQB_FLAG_REG_RETRY 32
 /* ints per node in the stats arrays of the batch API */
 #define BQB_NSTATS 9
 
@@ -232,11 +235,32 @@ size_t bqb_ipm_noalloc_bytes(int nodes, int nmax);
 int bqb_ipm_batch_noalloc(int nodes, int nmax, const int *n, const double *L, double *X, double *phi,
                           int *iters, int *status, void *stream);
 
-/* Tuning hook (no counterpart in the reference): force the IPM register
- * tile (P*100 + R, one of 1604-1607, 3204, 3205) for contexts and batches
- * created afterwards whose size it covers; 0 restores the size rule.
- * Returns -1 for an unknown tile. */
+/* IPM tile selection (no counterpart in the reference).  Subproblem sizes
+ * fall into BQB_TILE_CLASSES classes by their slab leading dimension
+ * (64, 80, 96, 112, 128, 160, 192, 224, 256); each class runs either its
+ * register tile (P*100 + R: 1604-1607, 3204-3208) or the blocked solve
+ * with the same slabs (ld*100 + 1), which keeps the matrices in their slabs
+ * and streams them through register micro-tiles (no spilled register
+ * tile; faster above n ~ 190).  Failed blocked solves are repeated with
+ * the register tile (BQB_FLAG_REG_RETRY).
+ * bqb_set_ipm_tile forces one tile for contexts and batches created
+ * afterwards whose size it covers (0 restores the table); -1 for an
+ * unknown tile. */
+#define BQB_TILE_CLASSES 9
 int bqb_set_ipm_tile(int tile);
+/* The per-class table (BQB_TILE_CLASSES entries): read, or install (each
+ * entry the class's register tile or ld*100 + 1; -1 otherwise). */
+int bqb_ipm_tile_table(int *tiles);
+int bqb_set_ipm_tile_table(const int *tiles);
+/* The tile autotuner: for every class whose slab size lies in
+ * [nmin, nmax] (rounded up to its class), times `nodes` synthetic
+ * subproblems of the class's largest size (Max-Cut L of G(n, 0.3) with
+ * weights +-1) with the register tile and with the blocked solve, `reps`
+ * launches each after one warm-up, and installs the faster in the table
+ * (it takes another only if that is at least 5 % faster, so near-ties keep
+ * the default).  ms_out (optional, 2 per class) receives the medians
+ * (register, blocked); tiles (optional) the resulting table. */
+int bqb_autotune_ipm(int nmin, int nmax, int nodes, int reps, double *ms_out, int *tiles);
 
 /* ---- batch: the full node bound (SDPbound) ----------------------------- */
 /* A context holds the device arenas of up to max_nodes B&B nodes of one

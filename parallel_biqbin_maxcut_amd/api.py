@@ -28,6 +28,10 @@ def declare(L):
     L.bqb_ipm_noalloc_bytes.restype = C.c_size_t
     L.bqb_ipm_batch_noalloc.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
     L.bqb_ipm_batch_noalloc.restype = C.c_int
+    L.bqb_ipm_tile_table.argtypes = [N.iptr]
+    L.bqb_set_ipm_tile_table.argtypes = [N.iptr]
+    L.bqb_autotune_ipm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, N.dptr, N.iptr]
+    L.bqb_ipm_tile_for.argtypes = [C.c_int]
     _declare_ctx(L)
     from . import learn
     learn.declare(L)
@@ -75,6 +79,34 @@ def ipm_batch(Ls, trace_cap=0):
         oy += n
     traces = [tr[b * 2 * trace_cap: b * 2 * trace_cap + 2 * iters[b]] for b in range(nodes)] if trace_cap else None
     return dict(X=Xs, phi=phi, iters=iters, trials=trials, status=status, y=ys, trace=traces)
+
+
+TILE_CLASS_LD = (64, 80, 96, 112, 128, 160, 192, 224, 256)
+
+
+def ipm_tile_table():
+    """{slab ld of the size class: IPM tile} (register tile P*100+R, or the
+    blocked solve ld*100+1)."""
+    t = np.zeros(len(TILE_CLASS_LD), dtype=np.int32)
+    N.lib().bqb_ipm_tile_table(_i(t))
+    return dict(zip(TILE_CLASS_LD, (int(v) for v in t)))
+
+
+def set_ipm_tile_table(table):
+    t = np.array([int(table[ld]) for ld in TILE_CLASS_LD], dtype=np.int32)
+    N.check(N.lib().bqb_set_ipm_tile_table(_i(t)))
+
+
+def autotune_ipm(nmin=1, nmax=256, nodes=64, reps=3):
+    """Time the register tile against the blocked solve for every size class
+    in [nmin, nmax] and install the faster (bqb_autotune_ipm).  Returns
+    (table, {ld: (register ms, blocked ms)})."""
+    ms = np.zeros(2 * len(TILE_CLASS_LD))
+    t = np.zeros(len(TILE_CLASS_LD), dtype=np.int32)
+    N.check(N.lib().bqb_autotune_ipm(nmin, nmax, nodes, reps, _d(ms), _i(t)))
+    timed = {ld: (round(float(ms[2 * c]), 4), round(float(ms[2 * c + 1]), 4))
+             for c, ld in enumerate(TILE_CLASS_LD) if ms[2 * c] > 0}
+    return dict(zip(TILE_CLASS_LD, (int(v) for v in t))), timed
 
 
 def mx_product(A, B, slices=3):
@@ -226,7 +258,7 @@ class SolveStats(C.Structure):
 
 NMAX = 1024
 NSTATS = 9   # ints per node in the stats arrays (BQB_NSTATS)
-FLAG_BUNDLE_FULL, FLAG_TRI_TRUNC, FLAG_PENT_TRUNC, FLAG_HEPT_TRUNC, FLAG_FP64_RETRY = 1, 2, 4, 8, 16
+FLAG_BUNDLE_FULL, FLAG_TRI_TRUNC, FLAG_PENT_TRUNC, FLAG_HEPT_TRUNC, FLAG_FP64_RETRY, FLAG_REG_RETRY = 1, 2, 4, 8, 16, 32
 
 
 class Problem(C.Structure):

@@ -33,6 +33,7 @@ extern "C" int bqb_set_ipm_tile(int tile)
     switch (tile) {
     case 0: case 1604: case 1605: case 1606: case 1607: case 3204: case 3205:
     case 3206: case 3207: case 3208:
+    case 6401: case 8001: case 9601: case 11201: case 12801: case 16001: case 19201: case 22401: case 25601:
         g_ipm_tile = tile;
         return 0;
     default:
@@ -40,9 +41,64 @@ extern "C" int bqb_set_ipm_tile(int tile)
     }
 }
 
+extern "C" int bqb_ipm_reg_tile_for_ld(int ld)
+{
+    switch (ld) {
+    case 64: return 1604;
+    case 80: return 1605;
+    case 96: return 1606;
+    case 112: return 1607;
+    case 128: return 3204;
+    case 160: return 3205;
+    case 192: return 3206;
+    case 224: return 3207;
+    case 256: return 3208;
+    default: return 0;
+    }
+}
+
+// Slab leading dimensions of the size classes, and the IPM tile chosen per
+// class: the register tile of that size, or the blocked solve with the same
+// slabs (tile ld*100 + 1).  Defaults: register tiles up to 192, the blocked
+// solve above (where the 32 x 7 and 32 x 8 register tiles spill); the tile
+// autotuner (bqb_autotune_ipm) or bqb_set_ipm_tile_table replace them.
+static const int k_class_ld[BQB_TILE_CLASSES] = {64, 80, 96, 112, 128, 160, 192, 224, 256};
+static int g_tile_table[BQB_TILE_CLASSES] = {1604, 1605, 1606, 1607, 3204, 3205, 3206, 22401, 25601};
+
+static int tile_class(int nmax)
+{
+    for (int c = 0; c < BQB_TILE_CLASSES; ++c)
+        if (nmax <= k_class_ld[c]) return c;
+    return -1;
+}
+
 extern "C" int bqb_ipm_tile_for(int nmax)
 {
     if (g_ipm_tile && (g_ipm_tile / 100) * (g_ipm_tile % 100) >= nmax) return g_ipm_tile;
+    const int c = tile_class(nmax);
+    return c < 0 || nmax < 1 ? 0 : g_tile_table[c];
+}
+
+extern "C" int bqb_ipm_tile_table(int *tiles)
+{
+    for (int c = 0; c < BQB_TILE_CLASSES; ++c) tiles[c] = g_tile_table[c];
+    return BQB_TILE_CLASSES;
+}
+
+extern "C" int bqb_set_ipm_tile_table(const int *tiles)
+{
+    for (int c = 0; c < BQB_TILE_CLASSES; ++c) {
+        const int ld = k_class_ld[c], t = tiles[c];
+        if (t != bqb_ipm_reg_tile_for_ld(ld) && t != ld * 100 + 1) return -1;
+    }
+    for (int c = 0; c < BQB_TILE_CLASSES; ++c) g_tile_table[c] = tiles[c];
+    return 0;
+}
+
+extern "C" int bqb_ipm_class_ld(int c) { return c >= 0 && c < BQB_TILE_CLASSES ? k_class_ld[c] : 0; }
+
+extern "C" int bqb_ipm_tile_reg_for(int nmax)
+{
     if (nmax <= 64) return 1604;
     if (nmax <= 80) return 1605;
     if (nmax <= 96) return 1606;
@@ -59,11 +115,11 @@ extern "C" int bqb_ipm_tile_for(int nmax)
 
 namespace {
 
-// Nodes whose mixed-precision solve failed (a factorization or a line search
-// gave up, or the iteration cap was hit): they are solved again with FP64
-// products.  flags (optional, stride in ints) receive BQB_FLAG_FP64_RETRY.
+// Nodes whose solve failed (a factorization or a line search gave up, or
+// the iteration cap was hit): they are solved again with FP64 products and
+// another tile.  flags (optional, stride in ints) receive `flag`.
 __global__ void ipm_retry_mask_kernel(int nodes, const int *active, const int *status, int *mask2, int *flags,
-                                      int stride)
+                                      int stride, int flag)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nodes) return;
@@ -71,16 +127,16 @@ __global__ void ipm_retry_mask_kernel(int nodes, const int *active, const int *s
     const int retry = (!active || active[b]) &&
                       (st == ERR_CHOL_Z || st == ERR_SCHUR || st == ERR_IPM_ITERS || st == ERR_LINESEARCH);
     mask2[b] = retry;
-    if (retry && flags) flags[(size_t)b * stride] |= BQB_FLAG_FP64_RETRY;
+    if (retry && flags) flags[(size_t)b * stride] |= flag;
 }
 
 }  // namespace
 
 extern "C" hipError_t bqb_launch_ipm_retry(const IpmBatchArgs *a, int nodes, int tile, int *mask2, int *flags,
-                                           int stride, hipStream_t s)
+                                           int stride, int flag, hipStream_t s)
 {
     hipLaunchKernelGGL(ipm_retry_mask_kernel, dim3((nodes + 255) / 256), dim3(256), 0, s, nodes, a->active,
-                       (const int *)a->status, mask2, flags, stride);
+                       (const int *)a->status, mask2, flags, stride, flag);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     IpmBatchArgs b = *a;
@@ -92,10 +148,14 @@ extern "C" hipError_t bqb_launch_ipm_retry(const IpmBatchArgs *a, int nodes, int
 
 extern "C" hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int tile, hipStream_t s)
 {
+    const int blocked = tile % 100 == 1;
     if (a->mix_slices > 0) {
         if (a->mix_slices < 2 || a->mix_slices > BQB_MIX_MAX_SLICES) return hipErrorInvalidValue;
-        return bqb_launch_ipm_mixed(a, nodes, tile, s);
+        // products on the matrix cores: the register-tile kernel with the
+        // same slab layout
+        return bqb_launch_ipm_mixed(a, nodes, blocked ? bqb_ipm_reg_tile_for_ld(tile / 100) : tile, s);
     }
+    if (blocked) return bqb_launch_ipm_blk(a, nodes, s);
     switch (tile) {
     case 1604: return launch_ipm<16, 4>(*a, nodes, s);
     case 1605: return launch_ipm<16, 5>(*a, nodes, s);

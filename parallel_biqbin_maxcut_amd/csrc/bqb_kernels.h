@@ -54,6 +54,7 @@ typedef struct IpmBatchArgs {
     unsigned long long *flops;  /* optional: += algorithmic FP64 flops of each node's solve */
     int mix_slices;        /* 0: FP64 products; 2..4: products on bf16 matrix cores with this many slices */
     unsigned long long *flops_mx;   /* optional (mixed): += bf16 matrix-core flops, 2 n^3 per slice pair */
+    int wslabs;            /* work slabs per node in `work` (the blocked solve needs 4) */
 } IpmBatchArgs;
 
 /* Algorithmic FP64 flops of one IPM solve of size n: per iteration two SPD
@@ -68,15 +69,24 @@ typedef struct IpmBatchArgs {
  * slice pairs, 2 n^3 each, per iteration. */
 #define BQB_IPM_FLOPS_MX(n, iters, s) ((double)(n) * (n) * (n) * 2.0 * (s) * ((s) + 1) * (iters))
 
-/* After a mixed-precision launch: solve again, with FP64 products, the
- * nodes whose mixed solve failed (mask2: [nodes] scratch; flags: optional
- * per-node flag words at the given int stride, BQB_FLAG_FP64_RETRY set). */
+/* The blocked solve for large subproblems (tile codes ld*100 + 1). */
+hipError_t bqb_launch_ipm_blk(const IpmBatchArgs *a, int nodes, hipStream_t s);
+/* The register tile whose slabs have leading dimension ld (a blocked tile
+ * code's register counterpart), 0 if none. */
+int bqb_ipm_reg_tile_for_ld(int ld);
+/* The register-tile size rule alone (no override, no tuned table). */
+int bqb_ipm_tile_reg_for(int nmax);
+/* After a mixed-precision or blocked launch: solve again, with FP64
+ * products and the given tile, the nodes whose solve failed (mask2:
+ * [nodes] scratch; flags: optional per-node flag words at the given int
+ * stride, which receive `flag`). */
 hipError_t bqb_launch_ipm_retry(const IpmBatchArgs *a, int nodes, int tile, int *mask2, int *flags, int stride,
-                                hipStream_t s);
+                                int flag, hipStream_t s);
 hipError_t bqb_launch_cert_ipm(const int *nvec, const double *L, const double *y, long ystride, double *work,
                                int ld, int wslabs, const int *status, double *out, int nodes, hipStream_t s);
 
 int bqb_ipm_tile_for(int nmax);
+int bqb_ipm_class_ld(int c);
 hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int tile, hipStream_t s);
 
 #ifdef __cplusplus

@@ -111,7 +111,8 @@ extern "C" int bqb_noalloc_setup(void)
 
 extern "C" size_t bqb_ipm_noalloc_bytes(int nodes, int nmax)
 {
-    const int ld = bqb_ipm_ld_for(nmax);
+    const int tile = bqb_ipm_tile_reg_for(nmax);
+    const int ld = (tile / 100) * (tile % 100);
     if (nodes < 1 || !ld) return 0;
     const size_t slab = sizeof(double) * (size_t)ld * ld;
     return 5 * slab * (size_t)nodes + ((sizeof(int) * (size_t)nodes + 255) & ~(size_t)255);
@@ -122,7 +123,9 @@ extern "C" int bqb_ipm_batch_noalloc(int nodes, int nmax, const int *n, const do
 {
     if (nodes < 1) { bqb_set_error("noalloc IPM: nodes = %d", nodes); return -1; }
     if (!n || !L || !X || !phi || !iters || !status) { bqb_set_error("noalloc IPM: null device pointer"); return -1; }
-    const int tile = bqb_ipm_tile_for(nmax);
+    // register tiles: their three work slabs per node are what the scratch
+    // layout holds
+    const int tile = bqb_ipm_tile_reg_for(nmax);
     if (nmax < 1 || !tile) { bqb_set_error("noalloc IPM: nmax = %d outside 1..256", nmax); return -1; }
     const size_t need = bqb_ipm_noalloc_bytes(nodes, nmax);
     if (need > (size_t)BQB_NOALLOC_SCRATCH_BYTES) {
@@ -154,6 +157,7 @@ extern "C" int bqb_ipm_batch_noalloc(int nodes, int nmax, const int *n, const do
     a.iters = iters;
     a.trials = trials;
     a.status = status;
+    a.wslabs = 3;
     if (bqb_check(bqb_launch_ipm_batch(&a, nodes, tile, s), "noalloc IPM launch")) return -1;
     hipLaunchKernelGGL(noalloc_unpack_kernel, dim3(nodes), dim3(PT), 0, s, n, (const double *)Xs, X, ld);
     return bqb_check(hipGetLastError(), "noalloc unpack launch");

@@ -152,7 +152,7 @@ static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, d
                           int trace_cap, int max_iters, double *work_out, int mix_slices, double *phi_cert,
                           double *ms_out)
 {
-    const int wslabs = BQB_IPM_WORK_SLABS(mix_slices);
+    int wslabs = BQB_IPM_WORK_SLABS(mix_slices);
     if (nodes <= 0) return 0;
     if (bqb_ensure_init()) return -1;
     int nmax = 0;
@@ -162,6 +162,7 @@ static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, d
     }
     int tile = bqb_ipm_tile_for(nmax);
     if (!tile) { bqb_set_error("n = %d exceeds the largest compiled IPM tile", nmax); return -1; }
+    if (tile % 100 == 1 && wslabs < 4) wslabs = 4;      /* the blocked solve's four work slabs */
     const int ld = bqb_ipm_ld_for(nmax);
     const size_t nn = (size_t)ld * ld;
     if (trace_cap < 0) trace_cap = 0;
@@ -219,12 +220,19 @@ static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, d
     a.max_iters = max_iters;
     a.dbg = work_out ? (double *)(d + off_dbg) : NULL;
     a.mix_slices = mix_slices;
+    a.wslabs = wslabs;
     hipEvent_t e0 = NULL, e1 = NULL;
     if (ms_out && (bqb_check(hipEventCreate(&e0), "event") || bqb_check(hipEventCreate(&e1), "event"))) return -1;
     if (ms_out) hipEventRecord(e0, 0);
     if (bqb_check(bqb_launch_ipm_batch(&a, nodes, tile, 0), "ipm_batch_kernel launch")) return -1;
     /* mixed solves that failed are repeated with FP64 products */
-    if (mix_slices && bqb_check(bqb_launch_ipm_retry(&a, nodes, tile, (int *)(d + off_m2), NULL, 0, 0), "retry launch"))
+    if (mix_slices &&
+        bqb_check(bqb_launch_ipm_retry(&a, nodes, tile, (int *)(d + off_m2), NULL, 0, 0, 0), "retry launch"))
+        return -1;
+    /* blocked solves that failed are repeated with the register tile */
+    if (!mix_slices && tile % 100 == 1 && !work_out &&
+        bqb_check(bqb_launch_ipm_retry(&a, nodes, bqb_ipm_reg_tile_for_ld(tile / 100), (int *)(d + off_m2), NULL, 0,
+                                       0, 0), "retry launch"))
         return -1;
     if (ms_out) hipEventRecord(e1, 0);
     if (phi_cert &&
@@ -265,7 +273,7 @@ static int ipm_batch_impl(int nodes, const int *n, const double *L, double *X, d
         free(hy);
     }
     if (work_out &&
-        bqb_check(hipMemcpy(work_out, d + off_W, sizeof(double) * wslabs * nn * nodes, hipMemcpyDeviceToHost), "D2H work"))
+        bqb_check(hipMemcpy(work_out, d + off_W, sizeof(double) * 3 * nn * nodes, hipMemcpyDeviceToHost), "D2H work"))
         return -1;
     if (work_out &&
         bqb_check(hipMemcpy(work_out + 3 * nn * nodes, d + off_dbg, sizeof(double) * 4 * ld * nodes, hipMemcpyDeviceToHost), "D2H dbg"))
@@ -378,5 +386,79 @@ out:
     if (e1) hipEventDestroy(e1);
     if (a) hipFree(a);
     if (b) hipFree(b);
+    return rc;
+}
+
+/* ---- IPM tile autotuner ------------------------------------------------ */
+static int cmp_double(const void *a, const void *b)
+{
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+/* Median launch time of the batch with the given tile forced. */
+static int time_tile(int tile, int nodes, const int *nv, const double *L, double *X, double *phi, int *it, int *tr,
+                     int *st, int reps, double *med)
+{
+    double ms[16];
+    if (reps > 16) reps = 16;
+    bqb_set_ipm_tile(tile);
+    for (int r = -1; r < reps; ++r) {
+        double t = 0.0;
+        if (ipm_batch_impl(nodes, nv, L, X, phi, it, tr, st, NULL, NULL, 0, 0, NULL, 0, NULL, &t)) {
+            bqb_set_ipm_tile(0);
+            return -1;
+        }
+        if (r >= 0) ms[r] = t;
+    }
+    bqb_set_ipm_tile(0);
+    qsort(ms, (size_t)reps, sizeof(double), cmp_double);
+    *med = ms[reps / 2];
+    return 0;
+}
+
+int bqb_autotune_ipm(int nmin, int nmax, int nodes, int reps, double *ms_out, int *tiles)
+{
+    if (nodes < 1 || reps < 1 || nmin < 1 || nmax < nmin) { bqb_set_error("autotune: bad arguments"); return -1; }
+    if (bqb_ensure_init()) return -1;
+    int table[BQB_TILE_CLASSES];
+    bqb_ipm_tile_table(table);
+    int rc = 0;
+    for (int c = 0; c < BQB_TILE_CLASSES && !rc; ++c) {
+        const int ld = bqb_ipm_class_ld(c), lo = c ? bqb_ipm_class_ld(c - 1) + 1 : 1;
+        if (ld < nmin || lo > nmax) continue;
+        /* the class's largest subproblems: G(ld, 0.3), weights +-1, as Max-Cut L */
+        const int n = ld;
+        double *adj = calloc((size_t)n * n, sizeof(double));
+        double *L = malloc(sizeof(double) * (size_t)n * n * nodes), *X = malloc(sizeof(double) * (size_t)n * n * nodes);
+        double *phi = malloc(sizeof(double) * nodes);
+        int *nv = malloc(sizeof(int) * nodes), *it = malloc(sizeof(int) * nodes), *tr = malloc(sizeof(int) * nodes),
+            *st = malloc(sizeof(int) * nodes);
+        if (!adj || !L || !X || !phi || !nv || !it || !tr || !st) { bqb_set_error("malloc"); rc = -1; }
+        uint64_t z = 0x9E3779B97F4A7C15ull * (uint64_t)(ld + 1);
+        for (int b = 0; b < nodes && !rc; ++b) {
+            for (int i = 0; i < n; ++i)
+                for (int j = i + 1; j < n; ++j) {
+                    z = z * 6364136223846793005ull + 1442695040888963407ull;
+                    const double w = ((z >> 33) % 10 < 3) ? (((z >> 20) & 1) ? 1.0 : -1.0) : 0.0;
+                    adj[i + (size_t)j * n] = adj[j + (size_t)i * n] = w;
+                }
+            if (bqb_build_L(n, adj, L + (size_t)b * n * n)) rc = -1;
+            nv[b] = n;
+        }
+        double mr = 0.0, mb = 0.0;
+        const int reg = bqb_ipm_reg_tile_for_ld(ld), blk = ld * 100 + 1;
+        if (!rc) rc = time_tile(reg, nodes, nv, L, X, phi, it, tr, st, reps, &mr);
+        if (!rc) rc = time_tile(blk, nodes, nv, L, X, phi, it, tr, st, reps, &mb);
+        if (!rc) {
+            const int cur = table[c], other = cur == reg ? blk : reg;
+            const double tcur = cur == reg ? mr : mb, toth = cur == reg ? mb : mr;
+            if (toth < 0.95 * tcur) table[c] = other;
+            if (ms_out) { ms_out[2 * c] = mr; ms_out[2 * c + 1] = mb; }
+        }
+        free(adj); free(L); free(X); free(phi); free(nv); free(it); free(tr); free(st);
+    }
+    if (!rc && bqb_set_ipm_tile_table(table)) { bqb_set_error("autotune: table rejected"); rc = -1; }
+    if (tiles) bqb_ipm_tile_table(tiles);
     return rc;
 }

@@ -398,6 +398,7 @@ static int ipm_launch(bqb_ctx *c, const double *L, double *X)
     a.status = c->p.ipm_status;
     a.active = c->p.mask;
     a.mix_slices = c->d.mix_slices;
+    a.wslabs = BQB_IPM_WORK_SLABS(1);     /* the arena's stride per slot */
     if (c->d.certify) {            /* the certificate needs y */
         a.y_out = c->p.ytest;
         a.ystride = c->d.ld;
@@ -409,9 +410,15 @@ static int ipm_launch(bqb_ctx *c, const double *L, double *X)
         hipEventRecord(c->pev[c->pev_n], c->stream);
     }
     if (bqb_check(bqb_launch_ipm_batch(&a, c->d.nodes, c->tile, c->stream), "ipm launch")) return -1;
+    /* failed solves again: mixed -> FP64 products, blocked -> register tile */
+    const int stride = (int)(sizeof(NodeState) / sizeof(int));
     if (a.mix_slices &&
-        bqb_check(bqb_launch_ipm_retry(&a, c->d.nodes, c->tile, c->p.mask2, &c->p.st[0].flags,
-                                       (int)(sizeof(NodeState) / sizeof(int)), c->stream), "ipm retry launch"))
+        bqb_check(bqb_launch_ipm_retry(&a, c->d.nodes, c->tile, c->p.mask2, &c->p.st[0].flags, stride,
+                                       BQB_FLAG_FP64_RETRY, c->stream), "ipm retry launch"))
+        return -1;
+    if (!a.mix_slices && c->tile % 100 == 1 &&
+        bqb_check(bqb_launch_ipm_retry(&a, c->d.nodes, bqb_ipm_reg_tile_for_ld(c->tile / 100), c->p.mask2,
+                                       &c->p.st[0].flags, stride, BQB_FLAG_REG_RETRY, c->stream), "ipm retry launch"))
         return -1;
     if (prof) {
         hipEventRecord(c->pev[c->pev_n + 1], c->stream);

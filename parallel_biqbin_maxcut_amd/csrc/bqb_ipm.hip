@@ -149,13 +149,11 @@ extern "C" hipError_t bqb_launch_ipm_retry(const IpmBatchArgs *a, int nodes, int
 extern "C" hipError_t bqb_launch_ipm_batch(const IpmBatchArgs *a, int nodes, int tile, hipStream_t s)
 {
     const int blocked = tile % 100 == 1;
+    if (blocked) return bqb_launch_ipm_blk(a, nodes, s);
     if (a->mix_slices > 0) {
         if (a->mix_slices < 2 || a->mix_slices > BQB_MIX_MAX_SLICES) return hipErrorInvalidValue;
-        // products on the matrix cores: the register-tile kernel with the
-        // same slab layout
-        return bqb_launch_ipm_mixed(a, nodes, blocked ? bqb_ipm_reg_tile_for_ld(tile / 100) : tile, s);
+        return bqb_launch_ipm_mixed(a, nodes, tile, s);
     }
-    if (blocked) return bqb_launch_ipm_blk(a, nodes, s);
     switch (tile) {
     case 1604: return launch_ipm<16, 4>(*a, nodes, s);
     case 1605: return launch_ipm<16, 5>(*a, nodes, s);

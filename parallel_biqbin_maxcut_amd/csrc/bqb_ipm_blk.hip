@@ -27,6 +27,7 @@
 
 #include "bqb_dist.h"
 #include "bqb_kernels.h"
+#include "bqb_mixed.h"
 
 namespace bqb {
 namespace {
@@ -382,6 +383,9 @@ __device__ void gemm_blk(double *C, const double *Zi, const double *X, const dou
 }
 
 // ---------------------------------------------------------------- the solve
+// MIX: the two products on the bf16 matrix cores (bqb_mixed.h, slices in
+// work slabs 4 and 5); everything else in FP64.
+template <bool MIX>
 __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n, BlkLds &S, const Tid &t)
 {
     const int ld = args.ld;
@@ -393,6 +397,8 @@ __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n,
     double *Wmi = W0 + nn;        // inv(Zi o X)
     double *Wdx = W0 + 2 * nn;    // dX1, then dX
     double *Ws = W0 + 3 * nn;     // products, line-search trial matrices
+    unsigned short *Wsl = (unsigned short *)(W0 + 4 * nn);   // bf16 slices of the product operands
+    const int mixs = MIX ? args.mix_slices : 0;
     const bool dbg = args.dbg != nullptr;
 
     // ---- initial point (ipm_mc_pk.c:65-99) -------------------------------
@@ -428,12 +434,19 @@ __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n,
     int iter = 0, trials = 0, status = 0;
     const int max_iters = args.max_iters > 0 ? args.max_iters : MAX_ITERS;
     bool zi_ready = false;
+    // mixed products while the iterate is far from optimal; FP64 products
+    // after the first iteration whose relative gap is below 1e-3, or, from
+    // the fourth on, whose primal or dual step was cut below 0.3
+    bool mx_on = MIX;
+    int iters_mx = 0;
     while (gap > 1e-2) {
         if (iter >= max_iters) {
             if (args.max_iters <= 0) status = ERR_IPM_ITERS;
             break;
         }
         ++iter;
+        const bool use_mx = MIX && mx_on;
+        iters_mx += use_mx ? 1 : 0;
 
         // ---- Zi = inv(Z), Z = Diag(zd) - offdiag(L) -------------------------
         if (!zi_ready) {
@@ -460,7 +473,16 @@ __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n,
         __syncthreads();
 
         // ---- dX1 = sym(Zi * (-Diag(dy1) X) - X) ---------------------------
-        gemm_blk<false>(Ws, Wzi, X, nullptr, S.dy1, nullptr, ld, n, S, t);
+        if (MIX && use_mx) {
+            // row i of Zi is its column i; column j of -Diag(dy1) X is
+            // -dy1 o X(:, j) (X symmetric)
+            const double *zi = Wzi;
+            const auto zi_line = [&](int i, int k) { return zi[k + (size_t)i * ld]; };
+            const auto t1_line = [&](int j, int k) { return -S.dy1[k] * X[k + (size_t)j * ld]; };
+            mx_product<NT>(zi_line, t1_line, n, mixs, Wsl, (int *)&S.pan[0][0], Ws, ld);
+        } else {
+            gemm_blk<false>(Ws, Wzi, X, nullptr, S.dy1, nullptr, ld, n, S, t);
+        }
         ew_pass<4>(
             n, t,
             [&](int i, int j, double *v) {
@@ -492,7 +514,16 @@ __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n,
         __syncthreads();
 
         // ---- dX = sym(Zi * (-Diag(dy2) X - Diag(dy1) dX1) + mu Zi + dX1) --
-        gemm_blk<true>(Ws, Wzi, X, Wdx, S.dy1, S.dy2, ld, n, S, t);
+        if (MIX && use_mx) {
+            const double *zi = Wzi;
+            const auto zi_line = [&](int i, int k) { return zi[k + (size_t)i * ld]; };
+            const auto t2_line = [&](int j, int k) {
+                return (-S.dy2[k] * X[k + (size_t)j * ld]) + (-S.dy1[k] * Wdx[k + (size_t)j * ld]);
+            };
+            mx_product<NT>(zi_line, t2_line, n, mixs, Wsl, (int *)&S.pan[0][0], Ws, ld);
+        } else {
+            gemm_blk<true>(Ws, Wzi, X, Wdx, S.dy1, S.dy2, ld, n, S, t);
+        }
         ew_pass<6>(
             n, t,
             [&](int i, int j, double *v) {
@@ -611,6 +642,7 @@ __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n,
         if (alpha_p + alpha_d > 1.6) mu *= 0.5;
         if (alpha_p + alpha_d > 1.9) mu *= 0.2;
         gap = fabs(phi - psi);
+        if (MIX && (gap < 1e-3 * fabs(phi) || (iter > 3 && (alpha_p < 0.3 || alpha_d < 0.3)))) mx_on = false;
     }
 
     if (t.tid == 0) {
@@ -618,19 +650,25 @@ __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n,
         args.iters[node] = iter;
         if (args.trials) args.trials[node] = trials;
         args.status[node] = status;
-        if (args.flops) atomicAdd(args.flops, (unsigned long long)BQB_IPM_FLOPS(n, iter, trials));
+        if (args.flops)
+            atomicAdd(args.flops, (unsigned long long)(MIX ? BQB_IPM_FLOPS_MIXED(n, iter, trials) +
+                                                                 BQB_IPM_FLOPS_PRODUCTS(n, iter - iters_mx)
+                                                           : BQB_IPM_FLOPS(n, iter, trials)));
+        if (MIX && args.flops_mx)
+            atomicAdd(args.flops_mx, (unsigned long long)BQB_IPM_FLOPS_MX(n, iters_mx, mixs));
     }
     if (args.y_out)
         for (int i = t.tid; i < n; i += NT) args.y_out[(size_t)node * args.ystride + i] = S.y[i];
 }
 
+template <bool MIX>
 __global__ __launch_bounds__(NT, 1) void ipm_blk_kernel(IpmBatchArgs args)
 {
     __shared__ BlkLds S;
     const int node = blockIdx.x;
     if (args.active && !args.active[node]) return;
     const int n = args.n[node];
-    if (n < 1 || n > NB || n > args.ld || args.wslabs < 4) {
+    if (n < 1 || n > NB || n > args.ld || args.wslabs < (MIX ? 6 : 4)) {
         if (threadIdx.x == 0) { args.status[node] = ERR_SIZE; args.iters[node] = 0; }
         return;
     }
@@ -640,17 +678,22 @@ __global__ __launch_bounds__(NT, 1) void ipm_blk_kernel(IpmBatchArgs args)
     t.tc = t.tid >> 4;
     t.lane = t.tid & 63;
     t.wave = t.tid >> 6;
-    blk_solve(args, node, n, S, t);
+    blk_solve<MIX>(args, node, n, S, t);
 }
 
 }  // namespace
 }  // namespace bqb
 
-// The blocked solve (FP64 products only).  Needs 4 work slabs per node
-// (args.wslabs >= 4) and n <= 256.
+// The blocked solve, n <= 256: FP64 products with 4 work slabs per node,
+// or the products on the matrix cores (a->mix_slices 2..4) with 6.
 extern "C" hipError_t bqb_launch_ipm_blk(const IpmBatchArgs *a, int nodes, hipStream_t s)
 {
-    if (a->wslabs < 4 || a->mix_slices) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bqb::ipm_blk_kernel, dim3(nodes), dim3(bqb::NT), 0, s, *a);
+    if (a->mix_slices) {
+        if (a->wslabs < 6 || a->mix_slices < 2 || a->mix_slices > bqb::MX_MAX_SLICES) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(bqb::ipm_blk_kernel<true>, dim3(nodes), dim3(bqb::NT), 0, s, *a);
+    } else {
+        if (a->wslabs < 4) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(bqb::ipm_blk_kernel<false>, dim3(nodes), dim3(bqb::NT), 0, s, *a);
+    }
     return hipGetLastError();
 }

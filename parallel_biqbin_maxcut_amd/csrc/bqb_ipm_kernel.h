@@ -223,12 +223,21 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
     // trial: that trial inverted exactly the next iteration's Z (same
     // diagonal zd + 1 * dy, same off-diagonal), so Zi is already in its slab
     bool zi_ready = false;
+    // mixed products while the iterate is far from optimal; FP64 products
+    // after the first iteration whose relative gap is below 1e-3, or, from
+    // the fourth on, whose primal or dual step was cut below 0.3 (near the
+    // optimum the slices' product error would dominate dX and stall the
+    // line searches)
+    bool mx_on = MIX;
+    int iters_mx = 0;
     while (gap > 1e-2) {
         if (iter >= max_iters) {
             if (args.max_iters <= 0) status = ERR_IPM_ITERS;
             break;
         }
         ++iter;
+        const bool use_mx = MIX && mx_on;
+        iters_mx += use_mx ? 1 : 0;
 
         // ---- Zi = inv(Z) --------------------------------------------------
         if (zi_ready) {
@@ -272,7 +281,7 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         __syncthreads();
 
         // ---- dX1 = sym(Zi * (-Diag(dy1) X) - X) ---------------------------
-        if constexpr (MIX) {
+        if (MIX && use_mx) {
             // row i of Zi is its column i; column j of -Diag(dy1) X is
             // -dy1 o X(:, j) (X symmetric)
             const auto zi_line = [&](int i, int k) { return Wzi[k + (size_t)i * ld]; };
@@ -323,7 +332,7 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         }
 
         // ---- dX = sym(Zi * (-Diag(dy2) X - Diag(dy1) dX1) + mu Zi + dX1) --
-        if constexpr (MIX) {
+        if (MIX && use_mx) {
             // the product goes to the Mi slab (Mi is not read again in this
             // iteration); dX1 stays in its slab for the sum below
             const auto zi_line = [&](int i, int k) { return Wzi[k + (size_t)i * ld]; };
@@ -450,6 +459,7 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         if (alpha_p + alpha_d > 1.6) mu *= 0.5;
         if (alpha_p + alpha_d > 1.9) mu *= 0.2;
         gap = fabs(phi - psi);
+        if (MIX && (gap < 1e-3 * fabs(phi) || (iter > 3 && (alpha_p < 0.3 || alpha_d < 0.3)))) mx_on = false;
     }
 
     if (t.tid == 0) {
@@ -458,9 +468,11 @@ __device__ __forceinline__ void ipm_solve(const IpmBatchArgs &args, const int no
         args.trials[node] = trials;
         args.status[node] = status;
         if (args.flops)
-            atomicAdd(args.flops, (unsigned long long)(MIX ? BQB_IPM_FLOPS_MIXED(n, iter, trials)
+            atomicAdd(args.flops, (unsigned long long)(MIX ? BQB_IPM_FLOPS_MIXED(n, iter, trials) +
+                                                                 BQB_IPM_FLOPS_PRODUCTS(n, iter - iters_mx)
                                                            : BQB_IPM_FLOPS(n, iter, trials)));
-        if (MIX && args.flops_mx) atomicAdd(args.flops_mx, (unsigned long long)BQB_IPM_FLOPS_MX(n, iter, mixs));
+        if (MIX && args.flops_mx)
+            atomicAdd(args.flops_mx, (unsigned long long)BQB_IPM_FLOPS_MX(n, iters_mx, mixs));
     }
     if (args.y_out)
         for (int I = t.tid; I < n; I += P * P) args.y_out[(size_t)node * args.ystride + I] = s_y[I];

@@ -65,6 +65,8 @@ typedef struct IpmBatchArgs {
 /* FP64 part with the products on the matrix cores: the two inversions. */
 #define BQB_IPM_FLOPS_MIXED(n, iters, trials) \
     ((double)(n) * (n) * (n) * (2.0 * (iters) + (trials) / 3.0))
+/* FP64 flops of the two products of `iters` iterations (4 n^3 each). */
+#define BQB_IPM_FLOPS_PRODUCTS(n, iters) ((double)(n) * (n) * (n) * 4.0 * (iters))
 /* bf16 matrix-core flops of one mixed solve: two products of s(s+1)/2
  * slice pairs, 2 n^3 each, per iteration. */
 #define BQB_IPM_FLOPS_MX(n, iters, s) ((double)(n) * (n) * (n) * 2.0 * (s) * ((s) + 1) * (iters))

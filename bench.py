@@ -170,9 +170,11 @@ def mixed_section(api, I, seed, nodes=64):
     """cfg3: Max-Cut on G(250, 0.1) with weights uniform in {-10..10} without 0:
     the root bound, then `nodes` B&B nodes with 0-40 random fixings (sizes
     210-250) bounded as one batch, with FP64 products and with the products
-    on the bf16 matrix cores (4 slices) and certified bounds.  The roofline
-    of the mixed IPM kernel: its bf16 matrix-core flops (2 n^3 per slice pair
-    and product) over the summed IPM launch time of the node batch."""
+    on the bf16 matrix cores (4 slices) and certified bounds, each with the
+    default IPM tile table (the blocked solve for these sizes) and with the
+    32 x 8 register tile forced (node batch only).  The roofline of the mixed
+    IPM kernel: its bf16 matrix-core flops (2 n^3 per slice pair and
+    product) over the summed IPM launch time of the node batch."""
     SP = api.maxcut_L(I.gnp_adj(250, 0.1, seed, weights="int10"))
     P = api.make_params()
     P.TriIneq = 2500                                  # adjust_TriIneq: 10 * n0
@@ -182,13 +184,20 @@ def mixed_section(api, I, seed, nodes=64):
     out = {"workload": f"G(250, 0.1) weights +-1..+-10, root bound then {nodes} nodes with 0-40 fixings "
                        "(sizes 210-250) as one batch; reference params"}
     res = {}
-    for mode, slices in (("fp64", 0), ("mixed", 4)):
+    out["ipm_tile_table"] = {str(k): v for k, v in api.ipm_tile_table().items()}
+    LB = diff = None
+    for mode, slices, tile in (("fp64", 0, 0), ("mixed", 4, 0), ("fp64_register_tile", 0, 3208),
+                               ("mixed_register_tile", 4, 3208)):
+        api.N.lib().bqb_set_ipm_tile(tile)
         ctx = api.BoundContext(SP, params=P, max_nodes=nodes)
+        api.N.lib().bqb_set_ipm_tile(0)
         ctx.set_precision(slices=slices, certify=1 if slices else 0)
-        t0 = time.perf_counter()
-        root = ctx.bound(z, z, LB=0.0, is_root=1, seeds=np.uint64(seed))
-        t_root = time.perf_counter() - t0
-        LB, diff = float(root["LB"][0]), float(root["diff"][0])
+        if not tile:
+            t0 = time.perf_counter()
+            root = ctx.bound(z, z, LB=0.0, is_root=1, seeds=np.uint64(seed))
+            t_root = time.perf_counter() - t0
+            if mode == "fp64":
+                LB, diff = float(root["LB"][0]), float(root["diff"][0])
         ctx.profile_ipm(max_launches=20000)
         ctx.sync()
         t0 = time.perf_counter()
@@ -201,22 +210,33 @@ def mixed_section(api, I, seed, nodes=64):
         if np.any(r["status"]):
             raise SystemExit(f"cfg3 {mode}: node failures {np.unique(r['status'])}")
         res[mode] = (r, dt, ipm_ms, fl64, flmx, launches)
-        out[f"{mode}_root_s"] = round(t_root, 3)
-        out[f"{mode}_root_bound"] = round(float(root["bound"][0]), 6)
+        if not tile:
+            out[f"{mode}_root_s"] = round(t_root, 3)
+            out[f"{mode}_root_bound"] = round(float(root["bound"][0]), 6)
         out[f"{mode}_bounds_per_s"] = round(nodes / dt, 3)
         out[f"{mode}_ipm_ms"] = round(ipm_ms, 2)
     bp, bm = res["fp64"][0]["bound"], res["mixed"][0]["bound"]
     rel = np.abs(bm - bp) / np.abs(bp)
     out["speedup_bounds_per_s"] = round(out["mixed_bounds_per_s"] / out["fp64_bounds_per_s"], 3)
     out["speedup_root"] = round(out["fp64_root_s"] / out["mixed_root_s"], 3)
+    out["speedup_register_tile_mixed_vs_fp64"] = round(
+        out["mixed_register_tile_bounds_per_s"] / out["fp64_register_tile_bounds_per_s"], 3)
+    out["speedup_blocked_vs_register_tile_fp64"] = round(
+        out["fp64_bounds_per_s"] / out["fp64_register_tile_bounds_per_s"], 3)
     out["rel_bound_diff_mixed_vs_fp64"] = {"median": float(np.median(rel)), "max": float(rel.max())}
     out["fp64_retries"] = int(np.sum(res["mixed"][0]["stats"][:, 8] & api.FLAG_FP64_RETRY > 0))
     _, _, ipm_ms, fl64, flmx, launches = res["mixed"]
     ach = flmx / (ipm_ms / 1e3) / 1e12 if ipm_ms > 0 else 0.0
-    out["roofline"] = {"kernel": "ipm_batch_kernel (products on bf16 matrix cores)", "bound": "mfma",
+    out["roofline"] = {"kernel": "ipm_blk_kernel (products on bf16 matrix cores)", "bound": "mfma",
                        "achieved": round(ach, 4), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": round(ach / BF16_PEAK_TFLOPS, 6), "traffic": None, "launches": launches,
                        "fp64_achieved": round(fl64 / (ipm_ms / 1e3) / 1e12, 4) if ipm_ms > 0 else 0.0}
+    # the FP64 blocked solve of the node batch against the FP64 peak
+    _, _, ipm_ms, fl64, _, launches = res["fp64"]
+    ach = fl64 / (ipm_ms / 1e3) / 1e12 if ipm_ms > 0 else 0.0
+    out["roofline_fp64"] = {"kernel": "ipm_blk_kernel (FP64)", "bound": "mfma", "achieved": round(ach, 4),
+                            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 5),
+                            "launches": launches}
     return out
 
 
@@ -274,16 +294,24 @@ def subproblem_matrices(api, SP, xf, sx):
     return out
 
 
-def ipm_isolated(api, I, seed, nodes, reps):
+def ipm_isolated(api, I, seed, nodes, reps, workload="cfg2", slices=0):
     """The batched IPM alone on one stream: the basic SDP of `nodes`
-    subproblems of the batched-bounding instance (sizes 60-100), one launch
-    per repetition, timed by HIP events around the launch."""
-    SP, _, xf, sx, _ = build_workload(api, I, seed, nodes)
+    subproblems of the batched-bounding instance (cfg2: G(100, 0.3) +-1,
+    sizes 60-100) or of the mixed-precision instance (cfg3: G(250, 0.1)
+    +-1..+-10, sizes 210-250), one launch per repetition, timed by HIP
+    events around the launch (and the FP64 repeat of failed solves)."""
+    if workload == "cfg3":
+        SP = api.maxcut_L(I.gnp_adj(250, 0.1, seed, weights="int10"))
+        xf, sx = I.random_fixings(250, 40, nodes, seed + 1)
+        desc = "G(250, 0.1) +-1..+-10"
+    else:
+        SP, _, xf, sx, _ = build_workload(api, I, seed, nodes)
+        desc = "G(100, 0.3) +-1"
     Ls = subproblem_matrices(api, SP, xf, sx)
-    api.ipm_batch_mixed(Ls, slices=0)                   # warm-up
+    api.ipm_batch_mixed(Ls, slices=slices)              # warm-up
     ms = []
     for _ in range(reps):
-        r = api.ipm_batch_mixed(Ls, slices=0)
+        r = api.ipm_batch_mixed(Ls, slices=slices)
         if np.any(r["status"]):
             raise SystemExit(f"IPM failures: {np.unique(r['status'], return_counts=True)}")
         ms.append(r["ms"])
@@ -293,8 +321,10 @@ def ipm_isolated(api, I, seed, nodes, reps):
     achieved = flops / (t / 1e3) / 1e12
     # algorithmic HBM bytes: L read and X written once per subproblem
     alg_bytes = float(np.sum(2 * 8 * ns ** 2))
-    return {"mode": "ipm_isolated", "kernel": "ipm_batch_kernel", "workload":
-            f"basic SDP of {nodes} subproblems of Max-Cut G(100, 0.3) +-1 with 0-40 fixings (sizes "
+    tiles = sorted({int(api.N.lib().bqb_ipm_tile_for(int(n))) for n in ns})
+    return {"mode": "ipm_isolated", "kernel": "ipm_blk_kernel" if all(t % 100 == 1 for t in tiles)
+            else "ipm_batch_kernel", "tiles": tiles, "slices": slices, "workload":
+            f"basic SDP of {nodes} subproblems of Max-Cut {desc} with 0-40 fixings (sizes "
             f"{int(ns.min())}-{int(ns.max())}), one launch on one stream", "launch_ms": round(t, 4),
             "launch_ms_all": [round(x, 4) for x in ms], "gflop_per_launch": round(flops / 1e9, 4),
             "mean_iters": round(float(r["iters"].mean()), 3), "mean_trials": round(float(r["trials"].mean()), 3),
@@ -322,6 +352,9 @@ def main():
                     help="1: launch trains as HIP graphs (basic sequence and outer-round slices)")
     ap.add_argument("--ipm-isolated", action="store_true",
                     help="time the batched IPM alone on --nodes subproblems, one launch per step (profiling)")
+    ap.add_argument("--ipm-workload", default="cfg2", choices=("cfg2", "cfg3"),
+                    help="instance of --ipm-isolated: cfg2 G(100, 0.3) or cfg3 G(250, 0.1)")
+    ap.add_argument("--ipm-slices", type=int, default=0, help="--ipm-isolated products: 0 FP64, 2..4 bf16 slices")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: bounding runs on one GPU; --gpus {args.gpus} is not supported "
@@ -331,7 +364,8 @@ def main():
     from parallel_biqbin_maxcut_amd import api, instances as I
 
     if args.ipm_isolated:
-        print(json.dumps(ipm_isolated(api, I, args.seed, args.nodes, max(1, args.steps))), flush=True)
+        print(json.dumps(ipm_isolated(api, I, args.seed, args.nodes, max(1, args.steps), args.ipm_workload,
+                                      args.ipm_slices)), flush=True)
         return
 
     SP, P, xf, sx, seeds = build_workload(api, I, args.seed, args.per_step)

@@ -33,8 +33,10 @@ port_over_reference_root_time ties it to the reference binary (measured on
 the same host type, one core each).
 
 Besides the headline, config carries: time_to_proof (G(60, 0.5) branch and
-bound to the proven optimum, one node at a time, 64 slots, and 64 slots with
-the mixed-precision certified bounds), cfg3_mixed (G(250, 0.1), FP64 against
+bound to the proven optimum: the plain FP64 form one node at a time, a
+64-slot node queue, children inheriting their parent's cuts and multipliers,
+bf16 matrix-core products with certified bounds, and the learned cut model;
+each form's speedup over the plain one), cfg3_mixed (G(250, 0.1), FP64 against
 bf16 matrix-core products with certified bounds, with the roofline of the
 mixed IPM against the bf16 peak) and cfg4_qubo (QUBO 150 as 151-vertex
 Max-Cut, 256 nodes per launch).
@@ -93,49 +95,71 @@ def cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample):
                       f"{dt:.1f} s wall on {cores} threads"}, bounds, status
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# time-to-proof forms: (name, bqb_solve_opt options); "plain" is the plain
+# FP64 GPU form, one node at a time, against which the others are compared
+PROOF_FORMS = (
+    ("plain", dict(batch=1)),
+    ("batched", dict(batch=64)),
+    ("batched_inherited", dict(batch=64, warm=True)),
+    ("batched_inherited_mixed", dict(batch=64, warm=True, slices=4)),
+    ("batched_inherited_mixed_learned", dict(batch=64, warm=True, slices=4, learned=True)),
+)
+
+
 def time_to_proof(api, I, seeds, cpu_seeds):
-    """Full B&B to the proven optimum, one node at a time (batch 1) and with
-    a 64-slot node queue, on G(60, 0.5) with unit weights, triangle cuts only
-    (the reference's params with pentagonal and heptagonal cuts off).  The
-    CPU oracle's sequential B&B runs on the first instances for comparison."""
+    """Full B&B to the proven optimum on G(60, 0.5) with unit weights,
+    triangle cuts only (the reference's params with pentagonal and
+    heptagonal cuts off), in the forms of PROOF_FORMS: one node at a time;
+    a 64-slot node queue; children inheriting their parent's cuts and
+    multipliers; bf16 matrix-core products with certified bounds; the
+    learned cut model.  Every form must prove the same optimum.  The CPU
+    oracle's sequential B&B runs on the first instances for comparison."""
     P = api.make_params(include_Pent=0, include_Hepta=0)
     out = {"workload": f"G(60, 0.5) unit weights, triangles only, seeds {seeds[0]}-{seeds[-1]}",
-           "optimum": [], "nodes_batch1": [], "seconds_batch1": [], "seconds_batch64": []}
-    out["seconds_batch64_mixed"] = []
-    out["seconds_batch64_mixed_learned"] = []
+           "forms": {name: {k: v for k, v in kw.items()} for name, kw in PROOF_FORMS},
+           "optimum": [], "nodes_plain": []}
     from parallel_biqbin_maxcut_amd import learn
     model = learn.MLP(params=np.load(os.path.join(ROOT, "parallel_biqbin_maxcut_amd", "models", "cut_model.npz"),
                                      allow_pickle=False)["params"])
+    secs = {name: [] for name, _ in PROOF_FORMS}
+    nodes = {name: 0 for name, _ in PROOF_FORMS}
     Ls = []
-    nodes64 = 0
     for sd in seeds:
         L = api.maxcut_L(I.gnp_adj(60, 0.5, sd, weights="unit"))
         Ls.append(L)
-        st1, _ = api.solve(L, params=P, seed=2020, batch=1)
-        st64, _ = api.solve(L, params=P, seed=2020, batch=64)
-        stm, _ = api.solve(L, params=P, seed=2020, batch=64, slices=4)
-        stl, _ = api.solve(L, params=P, seed=2020, batch=64, slices=4, model=model)
-        if st1.status or st64.status or stm.status or stl.status or not (st1.best == st64.best == stm.best == stl.best):
-            raise SystemExit(f"time-to-proof instance {sd}: status {st1.status}/{st64.status}/{stm.status}/"
-                             f"{stl.status}, optimum {st1.best} / {st64.best} / {stm.best} / {stl.best}")
-        out["optimum"].append(st1.best)
-        out["nodes_batch1"].append(int(st1.nodes))
-        out["seconds_batch1"].append(round(st1.seconds, 4))
-        out["seconds_batch64"].append(round(st64.seconds, 4))
-        out["seconds_batch64_mixed"].append(round(stm.seconds, 4))
-        out["seconds_batch64_mixed_learned"].append(round(stl.seconds, 4))
-        nodes64 += int(st64.nodes)
+        best = None
+        for name, kw in PROOF_FORMS:
+            kw = dict(kw)
+            if kw.pop("learned", False):
+                kw["model"] = model
+            st, _ = api.solve(L, params=P, seed=2020, **kw)
+            if st.status or (best is not None and st.best != best):
+                raise SystemExit(f"time-to-proof instance {sd}, form {name}: status {st.status}, "
+                                 f"optimum {st.best} (plain {best})")
+            best = st.best
+            secs[name].append(round(st.seconds, 4))
+            nodes[name] += int(st.nodes)
+            if name == "plain":
+                out["nodes_plain"].append(int(st.nodes))
+        out["optimum"].append(best)
+    progress(f"time to proof: {len(seeds)} instances in {len(PROOF_FORMS)} forms")
     model.close()
-    out["median_s_batch1"] = float(np.median(out["seconds_batch1"]))
-    out["median_s_batch64"] = float(np.median(out["seconds_batch64"]))
-    out["median_s_batch64_mixed"] = float(np.median(out["seconds_batch64_mixed"]))
-    out["median_s_batch64_mixed_learned"] = float(np.median(out["seconds_batch64_mixed_learned"]))
-    # the headline comparison: learned + batched + mixed against the plain
-    # FP64 form one node at a time, same instances, same proven optima
-    out["speedup_learned_batched_mixed_vs_plain"] = round(
-        sum(out["seconds_batch1"]) / sum(out["seconds_batch64_mixed_learned"]), 3)
+    for name, _ in PROOF_FORMS:
+        out[f"seconds_{name}"] = secs[name]
+        out[f"median_s_{name}"] = float(np.median(secs[name]))
+        out[f"total_s_{name}"] = round(float(np.sum(secs[name])), 4)
+        out[f"nodes_{name}"] = nodes[name]
+        # total time of the 20 proofs, plain form over this form
+        out[f"speedup_{name}_vs_plain"] = round(sum(secs["plain"]) / sum(secs[name]), 3)
+    best_form = max((name for name, _ in PROOF_FORMS), key=lambda nm: out[f"speedup_{nm}_vs_plain"])
+    out["fastest_form"] = best_form
     # bounds/s on the node stream of a real branch and bound (64 slots)
-    out["bab_node_bounds_per_s_batch64"] = round(nodes64 / sum(out["seconds_batch64"]), 2)
+    out["bab_node_bounds_per_s_batched"] = round(nodes["batched"] / sum(secs["batched"]), 2)
     if cpu_seeds:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
@@ -149,7 +173,7 @@ def time_to_proof(api, I, seeds, cpu_seeds):
             if ob.best != out["optimum"][i]:
                 raise SystemExit(f"time-to-proof instance {seeds[i]}: CPU oracle optimum {ob.best}, "
                                  f"GPU {out['optimum'][i]}")
-        out["cpu_oracle_seconds_batch1"] = cs
+        out["cpu_oracle_seconds_plain"] = cs
         out["cpu_oracle_cores"] = 1
     return out
 
@@ -377,9 +401,11 @@ def main():
     if root["status"][0]:
         raise SystemExit(f"root bound failed with status {root['status'][0]}")
     LB, diff = float(root["LB"][0]), float(root["diff"][0])
+    progress(f"root bound {float(root['bound'][0]):.4f}, incumbent {LB}")
 
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
+        progress(f"warm-up step {w + 1}/{args.warmup} done")
     # the IPM launch profile (event pairs around each IPM launch) keeps the
     # trains eager; with graphs the flops are counted in an untimed pass
     if not args.graphs:
@@ -391,6 +417,7 @@ def main():
         r = ctx.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
     ctx.sync()
     dt = time.perf_counter() - t0
+    progress(f"{args.steps} timed steps: {args.per_step * args.steps / dt:.1f} bounds/s")
     counters = ctx.counters()
     if args.graphs:
         ctx.set_graphs(False)
@@ -484,7 +511,9 @@ def main():
         line["config"]["time_to_proof"] = time_to_proof(api, I, list(range(1, 21)), 0 if args.no_cpu else 3)
     if not args.no_extra:
         line["config"]["cfg3_mixed"] = mixed_section(api, I, args.seed)
+        progress("mixed-precision section done")
         line["config"]["cfg4_qubo"] = qubo_section(api, I, args.seed)
+        progress("QUBO section done")
     if not args.no_cpu:
         sample = min(args.cpu_sample, args.per_step)
         cb, cbounds, cstatus = cpu_baseline(SP, P, xf, sx, LB, diff, seeds, sample)

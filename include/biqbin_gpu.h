@@ -81,6 +81,10 @@ typedef struct BiqBinParameters {
 /* An interior-point solve of the node by the blocked kernel (large
  * subproblems) failed and was repeated with the register-tile kernel. */
 #define BQB_FLAG_REG_RETRY 32
+/* The evaluation of the multipliers a node inherited from its parent failed
+ * (interior-point status); the node went on from gamma = 0 as the reference
+ * does. */
+#define BQB_FLAG_WARM_FAILED 64
 /* ints per node in the stats arrays of the batch API */
 #define BQB_NSTATS 9
 
@@ -188,6 +192,22 @@ typedef struct bqb_mlp bqb_mlp;
 int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed, int batch,
                  int max_bundle, int slices, int certify, const bqb_mlp *model, float drop, int *sol,
                  bqb_solve_stats *stats);
+/* All options of the node queue in one block (no counterpart in the
+ * reference).  warm = 1: every child node inherits its parent's final cuts
+ * whose multiplier is >= 1e-5 and that do not contain the newly fixed
+ * variable, with their multipliers (bqb_ctx_set_warm).  graphs: 1 launch
+ * trains as HIP graphs, 0 eager, -1 the library default. */
+typedef struct bqb_solve_options {
+    int batch;            /* node slots (>= 1) */
+    int max_bundle;       /* bundle capacity per node (<= 0: 64) */
+    int slices, certify;  /* bqb_ctx_set_precision */
+    const bqb_mlp *model; /* learned cut model or NULL */
+    float drop;
+    int warm;
+    int graphs;
+} bqb_solve_options;
+int bqb_solve_opt(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed,
+                  const bqb_solve_options *opt, int *sol, bqb_solve_stats *stats);
 
 /* ---- batch: interior-point solve of the basic SDP ---------------------- */
 /* Solves max <L_b, X_b> s.t. diag(X_b) = e, X_b psd for `nodes` independent
@@ -349,14 +369,16 @@ int bqb_ctx_trajectory(const bqb_ctx *c, int i, bqb_trajectory *out);
 void bqb_ctx_trajectory_clear(bqb_ctx *c);
 
 /* Stepwise use of a context (tests and diagnostics): load a batch into
- * slots 0..nodes-1, run one phase (0-14 control phases; 100 basic IPM,
+ * slots 0..nodes-1, run one phase (0-15 control phases; 100 basic IPM,
  * 101 IPM on L0, 102 heuristic,
  * 103 triangle update, 104/105 pentagonal/heptagonal update, 106 op_Bt
  * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP
  * (arg: an upper bound on every bundle size, 0 if unknown; a bound <= 16
  * selects the small-LDS QP kernel), 109 bundle step, 110/111 certificate of
  * the basic bound / of a new bundle center, 112/113 learned model on the
- * cuts of the first / of a round's separation), run the whole bound,
+ * cuts of the first / of a round's separation, 114 certificate of a
+ * warm-started center, 115 upload the nodes pushed since the last advance
+ * and run only their basic sequence), run the whole bound,
  * read results, and copy one node's
  * device buffer (ids documented in bqb_sdpbound.c) to or from the host. */
 int bqb_ctx_load(bqb_ctx *c, int nodes, const int *xfixed, const int *solX, const double *LB,
@@ -448,6 +470,28 @@ int bqb_ctx_ipm_stats(bqb_ctx *c, int *launches, double *ms_total, double *flops
  * are not used while the IPM profile is on or a host heuristic is set.
  * Off by default. */
 int bqb_ctx_set_graphs(bqb_ctx *c, int on);
+
+/* Inherited cuts (no counterpart in the reference, whose every node starts
+ * from no cuts, src/bounding.c:55-71).  With warm on (call while no node is
+ * in flight), a node pushed with bqb_ctx_push may be given a cut set right
+ * after the push (before the next advance): records as in bqb_fct_eval
+ * (tri 4 ints i > j > k, type 1-4; pentagonal / heptagonal 8 ints: type,
+ * then 5 or 7 distinct vertices, padded), vertices 0-based in the node's
+ * subproblem (its free variables in order, then the last vertex), with
+ * multipliers gamma >= 0 laid out [triangles | pentagonal | heptagonal].
+ * The node runs the reference's basic sequence (basic bound, heuristic,
+ * prune and give-up tests, first triangle separation, which keeps the
+ * inherited cuts with gamma >= 1e-5); its bundle then starts from the
+ * inherited multipliers, evaluated by fct_eval, when their value is below
+ * the basic bound (any gamma >= 0 gives a valid bound), else from gamma = 0
+ * as the reference's.  After every bqb_ctx_pop, bqb_ctx_last_cuts gives the
+ * popped node's final cut set and multipliers (empty if it was pruned or
+ * had none); the pointers stay valid until the next pop. */
+int bqb_ctx_set_warm(bqb_ctx *c, int on);
+int bqb_ctx_push_cuts(bqb_ctx *c, int slot, int ntri, const int *tri, int npent, const int *pent, int nhept,
+                      const int *hept, const double *gamma);
+int bqb_ctx_last_cuts(const bqb_ctx *c, int *ntri, const int **tri, int *npent, const int **pent, int *nhept,
+                      const int **hept, const double **gamma);
 /* The graph setting of contexts created afterwards (bqb_solve included);
  * returns the previous one. */
 int bqb_set_default_graphs(int on);

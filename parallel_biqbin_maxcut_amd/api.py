@@ -215,7 +215,7 @@ class NodeState(C.Structure):
         "n", "status", "active", "prune", "giveup", "done", "count", "bdl_iter", "NIneq",
         "NPentIneq", "NHeptaIneq", "bundle", "sep_tri", "sep_pent", "sep_hept", "heur_idx",
         "improved", "ipm_calls", "ipm_iters", "max_cuts", "max_bundle_seen", "is_root", "serious",
-        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags")] + [(f, C.c_double) for f in (
+        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags", "warm")] + [(f, C.c_double) for f in (
         "fixedvalue", "f", "f_test", "t", "bound", "basic_bound", "oldf", "LB", "diff", "viol3",
         "viol5", "gap", "del_", "lmax", "f_cert")] + [("seed", C.c_uint64)]
 
@@ -226,7 +226,7 @@ BUF = dict(state=0, L=1, L0=2, X=3, Xt=4, tri=5, tri_y=6, pent=7, pent_y=8, hept
            sa_val=29)
 PH = dict(CERT_BASIC=110, CERT_CENTER=111, SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OUTER_A=5, OUTER_B=6,
           SEP_PENT=7, SEP_HEPT=8, OUTER_C=9, REBUILD=10, FINISH=11, FCT_FINISH=12, LOAD=13,
-          BUNDLE_NEXT=14, IPM_BASIC=100, IPM_L0=101,
+          BUNDLE_NEXT=14, WARM=15, IPM_BASIC=100, IPM_L0=101, BASIC_TRAIN=115,
           HEUR=102, TRISEP=103, PENT=104, HEPT=105, CSR=106, OPBT=107, QP=108, STEP=109)
 
 
@@ -249,6 +249,12 @@ def _arr(ptr, count, dtype, shape=None):
     return a.reshape(shape) if shape is not None else a
 
 
+class SolveOptions(C.Structure):
+    """bqb_solve_options"""
+    _fields_ = [("batch", C.c_int), ("max_bundle", C.c_int), ("slices", C.c_int), ("certify", C.c_int),
+                ("model", C.c_void_p), ("drop", C.c_float), ("warm", C.c_int), ("graphs", C.c_int)]
+
+
 class SolveStats(C.Structure):
     _fields_ = [("best", C.c_double), ("root_bound", C.c_double), ("nodes", C.c_longlong),
                 ("batches", C.c_longlong), ("status", C.c_int), ("stopped", C.c_int),
@@ -259,6 +265,7 @@ class SolveStats(C.Structure):
 NMAX = 1024
 NSTATS = 9   # ints per node in the stats arrays (BQB_NSTATS)
 FLAG_BUNDLE_FULL, FLAG_TRI_TRUNC, FLAG_PENT_TRUNC, FLAG_HEPT_TRUNC, FLAG_FP64_RETRY, FLAG_REG_RETRY = 1, 2, 4, 8, 16, 32
+FLAG_WARM_FAILED = 64
 
 
 class Problem(C.Structure):
@@ -288,6 +295,8 @@ def _declare_host(L):
                                 N.iptr, C.POINTER(SolveStats)]
     L.bqb_solve_ex.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64, C.c_int, C.c_int,
                                C.c_int, C.c_int, C.c_void_p, C.c_float, N.iptr, C.POINTER(SolveStats)]
+    L.bqb_solve_opt.argtypes = [C.c_int, N.dptr, C.POINTER(BiqBinParameters), C.c_uint64,
+                                C.POINTER(SolveOptions), N.iptr, C.POINTER(SolveStats)]
     for f in ("Evaluate", "SDPbound"):
         getattr(L, f).argtypes = [C.POINTER(BabNode), C.POINTER(Problem), C.POINTER(Problem), C.c_int]
         getattr(L, f).restype = C.c_double
@@ -346,23 +355,20 @@ def evaluate_solution(SP_L, sol):
 
 
 def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certify=0, model=None, drop=0.0,
-          graphs=None):
-    """Branch and bound on the GPU (bqb_solve / bqb_solve_cap).  Returns
-    (stats, solution) with the solution over the first n0-1 vertices (the
-    last is on side 0).  graphs: launch trains as HIP graphs (None: the
-    library's default)."""
+          graphs=None, warm=False):
+    """Branch and bound on the GPU (bqb_solve_opt).  Returns (stats,
+    solution) with the solution over the first n0-1 vertices (the last is on
+    side 0).  graphs: launch trains as HIP graphs (None: the library's
+    default).  warm: children inherit their parent's final cuts and
+    multipliers."""
     SP_L = np.asfortranarray(SP_L, dtype=np.float64)
     n0 = SP_L.shape[0]
     p = params if params is not None else make_params()
     sol = np.zeros(n0 - 1, dtype=np.int32)
     st = SolveStats()
-    prev = set_default_graphs(graphs) if graphs is not None else None
-    try:
-        N.check(N.lib().bqb_solve_ex(n0, _d(SP_L), C.byref(p), seed, batch, max_bundle, slices, certify,
-                                     model.h if model is not None else None, drop, _i(sol), C.byref(st)))
-    finally:
-        if prev is not None:
-            set_default_graphs(prev)
+    o = SolveOptions(batch, max_bundle, slices, certify, model.h if model is not None else None, drop,
+                     1 if warm else 0, -1 if graphs is None else (1 if graphs else 0))
+    N.check(N.lib().bqb_solve_opt(n0, _d(SP_L), C.byref(p), seed, C.byref(o), _i(sol), C.byref(st)))
     return st, sol
 
 
@@ -395,6 +401,10 @@ def _declare_ctx(L):
                               N.iptr, N.iptr]
     L.bqb_ctx_busy.argtypes = [vp]
     L.bqb_ctx_capacity.argtypes = [vp]
+    L.bqb_ctx_set_warm.argtypes = [vp, C.c_int]
+    L.bqb_ctx_push_cuts.argtypes = [vp, C.c_int, C.c_int, N.iptr, C.c_int, N.iptr, C.c_int, N.iptr, N.dptr]
+    ipp = C.POINTER(N.iptr)
+    L.bqb_ctx_last_cuts.argtypes = [vp, N.iptr, ipp, N.iptr, ipp, N.iptr, ipp, C.POINTER(N.dptr)]
     L.bqb_ctx_load.argtypes = [vp, C.c_int, N.iptr, N.iptr, N.dptr, N.dptr, N.iptr, u64p]
     L.bqb_ctx_run.argtypes = [vp, C.c_int, C.c_int]
     L.bqb_ctx_bound.argtypes = [vp]
@@ -622,6 +632,69 @@ class BoundContext:
     def matrix(self, buf, node, n):
         a = self.get(buf, node, np.float64, self.ld * self.ld).reshape((self.ld, self.ld), order="F")
         return np.array(a[:n, :n], order="F")
+
+    # ---- incremental use: push / advance / pop, inherited cuts ----------
+    def set_warm(self, on=True):
+        for h in self.handles:
+            N.check(N.lib().bqb_ctx_set_warm(h, 1 if on else 0))
+
+    def push(self, xfixed, solX, LB=0.0, diff=0.0, is_root=0, seed=0, tag=0):
+        """bqb_ctx_push of one node; returns its slot."""
+        xf = np.ascontiguousarray(xfixed, dtype=np.int32)
+        sx = np.ascontiguousarray(solX, dtype=np.int32)
+        slot = N.lib().bqb_ctx_push(self.h, int(tag), _i(xf), _i(sx), float(LB), float(diff), int(is_root),
+                                    int(seed))
+        if slot < 0:
+            raise RuntimeError("bqb_ctx_push: " + N.lib().bqb_last_error().decode())
+        return slot
+
+    def push_cuts(self, slot, tris=(), pents=(), hepts=(), gamma=()):
+        """Inherited cuts of the node just pushed to `slot` (records as
+        fct_eval's: tri (i, j, k, type), pent (type, 5 vertices), hept (type,
+        7 vertices), vertices of the node's subproblem)."""
+        t = np.ascontiguousarray(np.reshape(np.asarray(tris, dtype=np.int32), (-1, 4)))
+        pn = np.zeros((len(pents), 8), np.int32)
+        for q, r in enumerate(pents):
+            pn[q, :6] = r
+        hp = np.zeros((len(hepts), 8), np.int32)
+        for q, r in enumerate(hepts):
+            hp[q, :8] = r
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        N.check(N.lib().bqb_ctx_push_cuts(self.h, int(slot), len(t), _i(t), len(pn), _i(pn), len(hp), _i(hp),
+                                          _d(g)))
+
+    def advance(self):
+        N.check(N.lib().bqb_ctx_advance(self.h))
+
+    def busy(self):
+        return N.lib().bqb_ctx_busy(self.h)
+
+    def pop(self):
+        """One finished node (dict) or None; 'cuts' holds its final cut set
+        (tri [k,4], pent [k,8], hept [k,8], gamma) in warm contexts."""
+        L = N.lib()
+        B = self.n0 - 1
+        tag = C.c_longlong()
+        bound, LBo, dfo = C.c_double(), C.c_double(), C.c_double()
+        imp, st = C.c_int(), C.c_int()
+        frac = np.zeros(B)
+        xb = np.zeros(B, np.int32)
+        stats = np.zeros(NSTATS, np.int32)
+        if not L.bqb_ctx_pop(self.h, C.byref(tag), C.byref(bound), _d(frac), _i(xb), C.byref(LBo), C.byref(imp),
+                             C.byref(dfo), C.byref(st), _i(stats)):
+            return None
+        nt, npn, nh = C.c_int(), C.c_int(), C.c_int()
+        pt, pp, ph = N.iptr(), N.iptr(), N.iptr()
+        pg = N.dptr()
+        L.bqb_ctx_last_cuts(self.h, C.byref(nt), C.byref(pt), C.byref(npn), C.byref(pp), C.byref(nh), C.byref(ph),
+                            C.byref(pg))
+        m = nt.value + npn.value + nh.value
+        cuts = dict(tri=np.ctypeslib.as_array(pt, (nt.value, 4)).copy() if nt.value else np.zeros((0, 4), np.int32),
+                    pent=np.ctypeslib.as_array(pp, (npn.value, 8)).copy() if npn.value else np.zeros((0, 8), np.int32),
+                    hept=np.ctypeslib.as_array(ph, (nh.value, 8)).copy() if nh.value else np.zeros((0, 8), np.int32),
+                    gamma=np.ctypeslib.as_array(pg, (m,)).copy() if m else np.zeros(0))
+        return dict(tag=tag.value, bound=bound.value, fracsol=frac, xbest=xb, LB=LBo.value, improved=imp.value,
+                    diff=dfo.value, status=st.value, stats=stats, cuts=cuts)
 
 
 if __name__ == "__main__":

@@ -160,13 +160,122 @@ double bqb_evaluate_solution(int n0, const double *SP_L, const int *sol)
 }
 
 /* ------------------------------------------------------------ node queue */
+/* Cuts a parent hands to its children: records over the original vertices
+ * (the last vertex n0-1 is the homogenising one), multipliers >= 1e-5;
+ * shared by both children. */
+typedef struct inherit {
+    int refs;
+    int ntri, npent, nhept;
+    int *rec;        /* ntri x 4 (i, j, k, type), then (npent + nhept) x 8 (type, vertices) */
+    double *y;       /* [ntri + npent + nhept] */
+} inherit;
+
 typedef struct qnode {
     double upper_bound;
     int level;
     uint64_t id;
     int *xfixed, *solX;
     double *fracsol;
+    inherit *inh;
 } qnode;
+
+static void inh_release(inherit *h)
+{
+    if (h && --h->refs == 0) { free(h->rec); free(h->y); free(h); }
+}
+
+static void node_free(qnode *x)
+{
+    if (x) inh_release(x->inh);
+    free(x);
+}
+
+/* The final cut set of the node just popped from the context (vertices in
+ * its subproblem: its free variables in order, then the last vertex) as
+ * records over the original vertices; only the cuts that survive the next
+ * purge (multiplier >= 1e-5, cutting_planes.c:151).  NULL if none. */
+static inherit *inh_from_popped(const bqb_ctx *ctx, const int *xfixed, int B, int *orig)
+{
+    int nt, np, nh;
+    const int *tri, *pent, *hept;
+    const double *y;
+    bqb_ctx_last_cuts(ctx, &nt, &tri, &np, &pent, &nh, &hept, &y);
+    if (nt + np + nh == 0) return NULL;
+    int nf = 0;
+    for (int i = 0; i < B; ++i)
+        if (!xfixed[i]) orig[nf++] = i;
+    orig[nf] = B;
+    inherit *h = calloc(1, sizeof *h);
+    if (!h) return NULL;
+    h->rec = malloc(sizeof(int) * (4 * (size_t)nt + 8 * (size_t)(np + nh)));
+    h->y = malloc(sizeof(double) * (size_t)(nt + np + nh));
+    if (!h->rec || !h->y) { free(h->rec); free(h->y); free(h); return NULL; }
+    h->refs = 1;
+    int q = 0;
+    for (int c = 0; c < nt; ++c) {
+        if (y[c] < 1e-5) continue;
+        int *r = h->rec + 4 * h->ntri;
+        r[0] = orig[tri[4 * c]]; r[1] = orig[tri[4 * c + 1]]; r[2] = orig[tri[4 * c + 2]]; r[3] = tri[4 * c + 3];
+        h->y[q++] = y[c];
+        h->ntri++;
+    }
+    for (int c = 0; c < np + nh; ++c) {
+        const double yc = y[nt + c];
+        if (yc < 1e-5) continue;
+        const int *src = c < np ? pent + 8 * c : hept + 8 * (c - np);
+        const int k = c < np ? 5 : 7;
+        int *r = h->rec + 4 * h->ntri + 8 * (h->npent + h->nhept);
+        r[0] = src[0];
+        for (int a = 1; a < 8; ++a) r[a] = a <= k ? orig[src[a]] : 0;
+        h->y[q++] = yc;
+        if (c < np) h->npent++; else h->nhept++;
+    }
+    if (q == 0) { inh_release(h); return NULL; }
+    return h;
+}
+
+/* Pushes the inherited cuts of x into the slot it was just pushed to:
+ * vertices renumbered into x's subproblem, cuts containing a variable that
+ * x fixes (and its parent did not) left out. */
+static int inh_push(bqb_ctx *ctx, int slot, const qnode *x, int B, int *pos)
+{
+    const inherit *h = x->inh;
+    int *tri = malloc(sizeof(int) * (4 * (size_t)h->ntri + 8 * (size_t)(h->npent + h->nhept) + 1));
+    double *y = malloc(sizeof(double) * (size_t)(h->ntri + h->npent + h->nhept + 1));
+    if (!tri || !y) { free(tri); free(y); bqb_set_error("node queue: out of memory"); return -1; }
+    int *kg = tri + 4 * h->ntri;
+    int nf = 0;
+    for (int i = 0; i < B; ++i) pos[i] = x->xfixed[i] ? -1 : nf++;
+    pos[B] = nf;
+    int nt = 0, np = 0, nh = 0, q = 0;
+    for (int c = 0; c < h->ntri; ++c) {
+        const int *r = h->rec + 4 * c;
+        const int a = pos[r[0]], b = pos[r[1]], d = pos[r[2]];
+        if (a < 0 || b < 0 || d < 0) continue;
+        tri[4 * nt] = a; tri[4 * nt + 1] = b; tri[4 * nt + 2] = d; tri[4 * nt + 3] = r[3];
+        y[q++] = h->y[c];
+        ++nt;
+    }
+    for (int pass = 0; pass < 2; ++pass) {       /* pentagonal, then heptagonal */
+        const int lo = pass ? h->npent : 0, hi = pass ? h->npent + h->nhept : h->npent, k = pass ? 7 : 5;
+        for (int c = lo; c < hi; ++c) {
+            const int *r = h->rec + 4 * h->ntri + 8 * c;
+            int *o = kg + 8 * (np + nh), ok = 1;
+            o[0] = r[0];
+            for (int a = 1; a < 8; ++a) {
+                o[a] = a <= k ? pos[r[a]] : 0;
+                if (a <= k && o[a] < 0) ok = 0;
+            }
+            if (!ok) continue;
+            y[q++] = h->y[h->ntri + c];
+            if (pass) ++nh; else ++np;
+        }
+    }
+    const int rc = (nt + np + nh == 0) ? 0 : bqb_ctx_push_cuts(ctx, slot, nt, tri, np, kg, nh, kg + 8 * np, y);
+    free(tri);
+    free(y);
+    return rc;
+}
 
 typedef struct qheap { qnode **d; long used, cap; } qheap;
 
@@ -216,6 +325,7 @@ static qnode *node_new(const qnode *parent, int B, uint64_t id)
     x->xfixed = (int *)(x->fracsol + B);
     x->solX = x->xfixed + B;
     x->id = id;
+    x->inh = NULL;
     memset(x->fracsol, 0, sizeof(double) * B);
     if (parent) {
         for (int i = 0; i < B; ++i) {
@@ -279,16 +389,43 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
                  int max_bundle, int slices, int certify, const bqb_mlp *model, float drop, int *sol,
                  bqb_solve_stats *out)
 {
+    const bqb_solve_options o = {batch, max_bundle, slices, certify, model, drop, 0, -1};
+    return bqb_solve_opt(n0, SP_L, params, seed, &o, sol, out);
+}
+
+/* The children of x (bab_functions.c:387-421 picks the variable), pushed
+ * on the heap; with inherited cuts both share x's final cut set. */
+static int branch(qheap *h, qnode *x, int B, int strategy, uint64_t *next_id, inherit *inh)
+{
+    const int ic = branching_variable(x, B, strategy);
+    for (int v = 0; v <= 1 && ic >= 0; ++v) {
+        qnode *c = node_new(x, B, (*next_id)++);
+        if (!c) return -1;
+        c->xfixed[ic] = 1;
+        c->solX[ic] = v;
+        if (inh) { c->inh = inh; inh->refs++; }
+        if (heap_push(h, c)) { node_free(c); return -1; }
+    }
+    return 0;
+}
+
+int bqb_solve_opt(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed,
+                  const bqb_solve_options *opt, int *sol, bqb_solve_stats *out)
+{
     memset(out, 0, sizeof *out);
     const double t0 = now_s();
+    int batch = opt->batch;
     if (batch < 1) batch = 1;
     const int B = n0 - 1;
-    bqb_ctx *ctx = bqb_ctx_create(n0, SP_L, params, batch, max_bundle, 0, 0, 0);
+    bqb_ctx *ctx = bqb_ctx_create(n0, SP_L, params, batch, opt->max_bundle, 0, 0, 0);
     if (!ctx) return -1;
-    if (bqb_ctx_set_precision(ctx, slices, certify) || bqb_ctx_set_model(ctx, model, drop)) {
+    const int warm = opt->warm;
+    if (bqb_ctx_set_precision(ctx, opt->slices, opt->certify) || bqb_ctx_set_model(ctx, opt->model, opt->drop) ||
+        (opt->graphs >= 0 && bqb_ctx_set_graphs(ctx, opt->graphs)) || (warm && bqb_ctx_set_warm(ctx, 1))) {
         bqb_ctx_destroy(ctx);
         return -1;
     }
+    int *vmap = malloc(sizeof(int) * (size_t)(B + 1));
     qheap h = {0};
     int rc = -1;
     uint64_t next_id = 0;
@@ -299,7 +436,7 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
     int *tag_free = malloc(sizeof(int) * batch), ntag = batch;
     double *frac = malloc(sizeof(double) * B);
     int *xb = malloc(sizeof(int) * B);
-    if (!inflight || !tag_free || !frac || !xb) { bqb_set_error("calloc"); goto done; }
+    if (!inflight || !tag_free || !frac || !xb || !vmap) { bqb_set_error("calloc"); goto done; }
     for (int i = 0; i < batch; ++i) tag_free[i] = batch - 1 - i;
 
     /* root node (bab_functions.c:39-69) */
@@ -332,15 +469,12 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
         out->root_bound = bound;
         memcpy(root->fracsol, frac, sizeof(double) * B);
         if (LB + 1.0 < root->upper_bound && !params->root) {
-            const int ic = branching_variable(root, B, params->branchingStrategy);
-            for (int v = 0; v <= 1 && ic >= 0; ++v) {
-                qnode *c = node_new(root, B, next_id++);
-                if (!c || heap_push(&h, c)) { free(c); free(root); goto done; }
-                c->xfixed[ic] = 1;
-                c->solX[ic] = v;
-            }
+            inherit *inh = warm ? inh_from_popped(ctx, root->xfixed, B, vmap) : NULL;
+            const int r = branch(&h, root, B, params->branchingStrategy, &next_id, inh);
+            inh_release(inh);
+            if (r) { node_free(root); goto done; }
         }
-        free(root);
+        node_free(root);
     }
 
     /* node queue: refill free slots with the best open nodes after every
@@ -350,12 +484,15 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
         if (h.used > out->max_queue) out->max_queue = (int)h.used;
         while (ntag > 0 && h.used > 0) {
             qnode *x = heap_pop(&h);
-            if (!(LB + 1.0 < x->upper_bound)) { free(x); continue; }   /* cannot beat the incumbent */
+            if (!(LB + 1.0 < x->upper_bound)) { node_free(x); continue; }   /* cannot beat the incumbent */
             const int t = tag_free[--ntag];
-            if (bqb_ctx_push(ctx, t, x->xfixed, x->solX, LB, diff, 0, bqb_key(seed, x->id)) < 0) {
-                free(x);
+            const int slot = bqb_ctx_push(ctx, t, x->xfixed, x->solX, LB, diff, 0, bqb_key(seed, x->id));
+            if (slot < 0 || (x->inh && inh_push(ctx, slot, x, B, vmap))) {
+                node_free(x);
                 goto done;
             }
+            inh_release(x->inh);         /* the cuts are staged in the context */
+            x->inh = NULL;
             inflight[t] = x;
         }
         if (bqb_ctx_busy(ctx) == 0) break;
@@ -381,25 +518,21 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
             x->upper_bound = bound;
             memcpy(x->fracsol, frac, sizeof(double) * B);
             if (!out->status && !fail && LB + 1.0 < x->upper_bound) {
-                const int ic = branching_variable(x, B, params->branchingStrategy);
-                for (int v = 0; v <= 1 && ic >= 0; ++v) {
-                    qnode *c = node_new(x, B, next_id++);
-                    if (!c || heap_push(&h, c)) { free(c); fail = 1; break; }
-                    c->xfixed[ic] = 1;
-                    c->solX[ic] = v;
-                }
+                inherit *inh = warm ? inh_from_popped(ctx, x->xfixed, B, vmap) : NULL;
+                if (branch(&h, x, B, params->branchingStrategy, &next_id, inh)) fail = 1;
+                inh_release(inh);
             }
-            free(x);
+            node_free(x);
         }
         if (fail || out->status) goto done;
     }
     rc = 0;
 done:
-    while (h.used > 0) free(heap_pop(&h));
+    while (h.used > 0) node_free(heap_pop(&h));
     free(h.d);
     if (inflight)
-        for (int i = 0; i < batch; ++i) free(inflight[i]);
-    free(inflight); free(tag_free); free(frac); free(xb);
+        for (int i = 0; i < batch; ++i) node_free(inflight[i]);
+    free(inflight); free(tag_free); free(frac); free(xb); free(vmap);
     bqb_ctx_destroy(ctx);
     out->best = LB;
     out->seconds = now_s() - t0;

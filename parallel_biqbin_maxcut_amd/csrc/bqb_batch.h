@@ -44,6 +44,8 @@ typedef struct NodeState {
     int bdl_left;       /* bundle iterations left in the current outer round */
     int rphase;         /* outer round: 0 to begin, 1 bundle iterations, 2 round end */
     int flags;          /* BQB_FLAG_* events of the node (reported in stats[8]) */
+    int warm;           /* loaded with inherited cuts and multipliers: its first
+                           bundle center is evaluated at them (until done) */
     double fixedvalue, f, f_test, t, bound, basic_bound, oldf;
     double LB, diff, viol3, viol5, gap, del, lmax;
     double f_cert;      /* certified upper bound of f (when certifying) */

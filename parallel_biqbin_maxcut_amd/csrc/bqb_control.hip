@@ -166,6 +166,19 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             const int i = e % n, j = e / n;
             L[i + (size_t)j * d.ld] *= 0.25;
         }
+        // A node loaded with inherited cuts (records and multipliers already
+        // in its cut arrays) keeps them: the multipliers become the current
+        // gamma, laid out [triangles | pentagonal | heptagonal], which the
+        // first triangle update reads for its purge.
+        const int warm = sp->warm;
+        const int wt = warm ? sp->NIneq : 0, wp = warm ? sp->NPentIneq : 0, wh = warm ? sp->NHeptaIneq : 0;
+        if (warm) {
+            double *gam = p.gamma + (size_t)node * d.max_m;
+            for (int c = threadIdx.x; c < wt; c += CT) gam[c] = p.tri_y[(size_t)node * d.max_tri + c];
+            for (int c = threadIdx.x; c < wp; c += CT) gam[wt + c] = p.pent_y[(size_t)node * d.max_pent + c];
+            for (int c = threadIdx.x; c < wh; c += CT) gam[wt + wp + c] = p.hept_y[(size_t)node * d.max_hept + c];
+        }
+        __syncthreads();
         if (threadIdx.x == 0) {
             sp->n = n;
             sp->fixedvalue = fv;
@@ -174,7 +187,9 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             sp->prune = sp->giveup = sp->done = 0;
             sp->count = 0;
             sp->bdl_iter = P.init_bundle_iter;
-            sp->NIneq = sp->NPentIneq = sp->NHeptaIneq = 0;
+            sp->NIneq = wt;
+            sp->NPentIneq = wp;
+            sp->NHeptaIneq = wh;
             sp->bundle = 0;
             sp->heur_idx = 0;
             sp->improved = 0;
@@ -199,7 +214,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         switch (phase) {
         case PH_POST_BASIC: case PH_CHECK_BASIC: case PH_BUNDLE_INIT: want = BQB_STAGE_NEW; break;
         case PH_OUTER_BEGIN: case PH_OUTER_A: case PH_OUTER_B: case PH_SEP_PENT: case PH_SEP_HEPT:
-        case PH_OUTER_C: case PH_REBUILD: case PH_BUNDLE_NEXT: want = BQB_STAGE_LOOP; break;
+        case PH_OUTER_C: case PH_REBUILD: case PH_BUNDLE_NEXT: case PH_WARM: want = BQB_STAGE_LOOP; break;
         default: break;
         }
         // outer rounds advance per node: begin (rphase 0), bundle
@@ -209,7 +224,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         case PH_OUTER_BEGIN: want_r = 0; break;
         case PH_BUNDLE_NEXT: want_r = 1; break;
         case PH_OUTER_A: case PH_OUTER_B: case PH_SEP_PENT: case PH_SEP_HEPT: case PH_OUTER_C:
-        case PH_REBUILD: want_r = 2; break;
+        case PH_REBUILD: case PH_WARM: want_r = 2; break;
         default: break;
         }
         if ((want >= 0 && s.stage != want) || (want_r >= 0 && s.rphase != want_r)) {
@@ -220,6 +235,11 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
 
     if (phase == PH_BUNDLE_NEXT) {
         if (threadIdx.x == 0) p.mask[node] = (s.active && s.bdl_left > 0) ? 1 : 0;
+        return;
+    }
+
+    if (phase == PH_WARM) {
+        if (threadIdx.x == 0) p.mask[node] = (s.active && s.warm) ? 1 : 0;
         return;
     }
 
@@ -262,12 +282,16 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
     if (phase == PH_BUNDLE_INIT) {
         // bounding.c:127-163
         if (!s.active) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
-        const int m = s.NIneq;
+        // m = NIneq unless the node inherited pentagonal / heptagonal cuts
+        const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
         double *gam = p.gamma + (size_t)node * d.max_m;
         double *g = p.g + (size_t)node * d.max_m;
         double *G = p.G + (size_t)node * d.max_bundle * d.max_m;
         const double *X = p.X + node * slab;
-        for (int c = threadIdx.x; c < m; c += CT) gam[c] = p.tri_y[(size_t)node * d.max_tri + c];
+        for (int c = threadIdx.x; c < s.NIneq; c += CT) gam[c] = p.tri_y[(size_t)node * d.max_tri + c];
+        for (int c = threadIdx.x; c < s.NPentIneq; c += CT) gam[s.NIneq + c] = p.pent_y[(size_t)node * d.max_pent + c];
+        for (int c = threadIdx.x; c < s.NHeptaIneq; c += CT)
+            gam[s.NIneq + s.NPentIneq + c] = p.hept_y[(size_t)node * d.max_hept + c];
         subgradient(d, p, node, s, X, g);
         __syncthreads();
         for (int c = threadIdx.x; c < m; c += CT) G[c] = g[c];
@@ -284,7 +308,9 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             sp->t = 0.5 * (s.bound - s.LB) / (m * s.viol3 * s.viol3);
             sp->cuts_changed = 1;
             sp->stage = BQB_STAGE_LOOP;
-            sp->rphase = 0;
+            // an inherited node evaluates its multipliers (PH_WARM, op_Bt,
+            // IPM, PH_REBUILD) before its first outer round
+            sp->rphase = s.warm ? 2 : 0;
             p.mask[node] = 1;
         }
         return;
@@ -405,7 +431,8 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
     if (phase == PH_REBUILD) {
         // bounding.c:278-321: extend the bundle with fct_eval(gamma) after a
         // cut change; G is recomputed for the new cut set.
-        if (!s.active) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
+        // arg 1: the evaluation of inherited multipliers (warm nodes only)
+        if (!s.active || (arg == 1 && !s.warm)) { if (threadIdx.x == 0) p.mask[node] = 0; return; }
         const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
         if (s.bundle >= d.max_bundle) {
             if (threadIdx.x == 0) bundle_full_giveup(sp, s, p.mask + node, d.certify);
@@ -434,22 +461,46 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         // With warm-started multipliers the new stability center gamma is no
         // longer (nearly) the old one, so its value f(gamma) = phi + sum(gamma)
         // from this evaluation becomes f (any gamma >= 0 gives a valid bound).
+        // An inherited node's first center: f(gamma) of the inherited
+        // multipliers if it is below the basic value f(0), else gamma = 0 as
+        // in the reference (the evaluation stays in the bundle either way).
         double sg = 0.0;
-        if (d.learned) {
-            const double *gam = p.gamma + (size_t)node * d.max_m;
+        double *gam = p.gamma + (size_t)node * d.max_m;
+        if (d.learned || s.warm) {
             for (int c = threadIdx.x; c < m; c += CT) sg += gam[c];
             sg = block_sum(sg, red);
         }
+        const double fnew = p.phi[node] + sg;
+        const bool keep_center = !s.warm || (fnew < s.f && !p.ipm_status[node]);
+        if (!keep_center)
+            for (int c = threadIdx.x; c < m; c += CT) gam[c] = 0.0;
+        __syncthreads();
         if (threadIdx.x == 0) {
-            if (d.learned) sp->f = p.phi[node] + sg;
-            p.F[(size_t)node * d.max_bundle + s.bundle] = Fb;
-            sp->bundle = s.bundle + 1;
-            if (sp->bundle > s.max_bundle_seen) sp->max_bundle_seen = sp->bundle;
-            sp->t = s.t * 1.05;
+            const int failed = p.ipm_status[node];
             sp->ipm_calls += 1;
             sp->ipm_iters += p.ipm_iters[node];
-            if (p.ipm_status[node]) { sp->status = p.ipm_status[node]; sp->active = 0; }
             sp->rphase = 0;
+            if (s.warm) {
+                // a failed evaluation of inherited multipliers adds nothing
+                // and leaves the node on its reference path (gamma = 0)
+                if (failed) sp->flags = s.flags | BQB_FLAG_WARM_FAILED;
+                else {
+                    if (keep_center) sp->f = fnew;
+                    p.F[(size_t)node * d.max_bundle + s.bundle] = Fb;
+                    sp->bundle = s.bundle + 1;
+                    if (sp->bundle > s.max_bundle_seen) sp->max_bundle_seen = sp->bundle;
+                }
+                sp->warm = 0;
+                // a certificate of the new center follows only where it moved
+                p.mask[node] = keep_center ? 1 : 0;
+            } else {
+                if (d.learned) sp->f = fnew;
+                p.F[(size_t)node * d.max_bundle + s.bundle] = Fb;
+                sp->bundle = s.bundle + 1;
+                if (sp->bundle > s.max_bundle_seen) sp->max_bundle_seen = sp->bundle;
+                sp->t = s.t * 1.05;
+                if (failed) { sp->status = failed; sp->active = 0; }
+            }
         }
         return;
     }

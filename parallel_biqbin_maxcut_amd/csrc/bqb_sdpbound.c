@@ -120,6 +120,21 @@ struct bqb_ctx {
     int capturing;             /* phases go into a capture, not counted as eager */
     int cap_phases;
     long long cnt[BQB_NCOUNTERS];
+    /* inherited cuts (bqb_ctx_set_warm): the cut sets attached to staged
+     * nodes until they are uploaded, and the final cut set of the node
+     * popped last */
+    int warm;
+    struct cut_pack **wstage;  /* [max_nodes] by slot, or NULL */
+    struct cut_pack *pop_cuts;
+};
+
+/* A node's cut set: records in the device layout and the multipliers
+ * [triangles | pentagonal | heptagonal]; one malloc'd block. */
+struct cut_pack {
+    int ntri, npent, nhept;
+    TriCut *tri;
+    KgonCut *pent, *hept;
+    double *y;
 };
 
 struct done_node {
@@ -127,7 +142,24 @@ struct done_node {
     NodeState st;
     double *frac;    /* n0-1 */
     int *xb;         /* n0-1 */
+    struct cut_pack *cuts;   /* final cut set (warm contexts, branched nodes) */
 };
+
+static struct cut_pack *pack_alloc(int ntri, int npent, int nhept)
+{
+    const size_t bytes = sizeof(struct cut_pack) + sizeof(TriCut) * (size_t)ntri
+                         + sizeof(KgonCut) * (size_t)(npent + nhept) + sizeof(double) * (size_t)(ntri + npent + nhept);
+    struct cut_pack *k = malloc(bytes);
+    if (!k) { bqb_set_error("cut set: out of memory"); return NULL; }
+    k->ntri = ntri;
+    k->npent = npent;
+    k->nhept = nhept;
+    k->y = (double *)(k + 1);
+    k->tri = (TriCut *)(k->y + ntri + npent + nhept);
+    k->pent = (KgonCut *)(k->tri + ntri);
+    k->hept = k->pent + npent;
+    return k;
+}
 
 #define CK(call, what) do { if (bqb_check((call), (what))) goto fail; } while (0)
 
@@ -166,8 +198,12 @@ static void free_host(bqb_ctx *c)
     free(c->tr_f); free(c->tr_t); free(c->tr_bound); free(c->tr_cuts); free(c->tr_n);
     free(c->h_st); free(c->tag); free(c->free_stack); free(c->of_slot); free(c->ld_st);
     free(c->ld_xf); free(c->ld_sx); free(c->h_frac); free(c->h_xb);
-    for (int i = 0; i < c->done_cap; ++i) { free(c->done[i].frac); free(c->done[i].xb); }
+    for (int i = 0; i < c->done_cap; ++i) { free(c->done[i].frac); free(c->done[i].xb); free(c->done[i].cuts); }
     free(c->done);
+    if (c->wstage)
+        for (int i = 0; i < c->max_nodes; ++i) free(c->wstage[i]);
+    free(c->wstage);
+    free(c->pop_cuts);
 }
 
 /* All slots free, nothing staged, no finished nodes pending. */
@@ -182,7 +218,10 @@ static void engine_reset(bqb_ctx *c)
     c->nload = 0;
     c->loaded = 0;
     c->hi = 0;
+    for (int i = 0; i < c->done_cap; ++i) { free(c->done[i].cuts); c->done[i].cuts = NULL; }
     c->ndone = c->done_head = 0;
+    if (c->wstage)
+        for (int i = 0; i < c->max_nodes; ++i) { free(c->wstage[i]); c->wstage[i] = NULL; }
     if (c->tr_n) memset(c->tr_n, 0, sizeof(int) * c->max_nodes);
 }
 
@@ -492,6 +531,9 @@ int bqb_ctx_set_heuristic(bqb_ctx *c, bqb_heuristic_fn fn, void *user)
 }
 
 /* Run one phase kernel on the current batch (also used by tests). */
+static int upload_staged(bqb_ctx *c);
+static int basic_train(bqb_ctx *c);
+
 int bqb_ctx_run(bqb_ctx *c, int what, int arg)
 {
     const BatchDims *d = &c->d;
@@ -515,13 +557,22 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
     case 110: return bqb_check(bqb_launch_cert_node(d, p, 0, s), "certificate launch");
     case 111: return bqb_check(bqb_launch_cert_node(d, p, 1, s), "certificate launch");
     case 114: return bqb_check(bqb_launch_cert_node(d, p, 2, s), "certificate launch");
+    case 115: {
+        /* the nodes pushed since the last advance: upload and run their
+         * basic sequence only (stepwise inspection of a node's start) */
+        if (c->nload && upload_staged(c)) return -1;
+        if (c->hi == 0) return 0;
+        c->d.nodes = c->hi;
+        c->loaded = 0;
+        return basic_train(c);
+    }
     case 112: case 113:
         if (!c->model && !c->rec_cap) return 0;
         return bqb_check(bqb_launch_learn(d, p, c->model, c->drop, c->rec_cap ? c->d_rec_count : NULL, c->rec_cap,
                                           c->d_rec_feat, c->d_rec_key, c->d_lidx, c->d_lkeep, what == 113, s),
                          "learn launch");
     default:
-        if (what >= 0 && what <= PH_BUNDLE_NEXT)
+        if (what >= 0 && what <= PH_WARM)
             return bqb_check(bqb_launch_control(d, p, what, arg, s), "control launch");
         bqb_set_error("unknown phase %d", what);
         return -1;
@@ -580,11 +631,112 @@ static int upload_staged(bqb_ctx *c)
     if (bqb_check(hipMemcpyAsync(c->p.ld_st, c->ld_st, sizeof(NodeState) * c->nload, hipMemcpyHostToDevice, s), "H2D staged state")) return -1;
     if (bqb_check(hipMemcpyAsync(c->p.ld_xf, c->ld_xf, sizeof(int) * B * c->nload, hipMemcpyHostToDevice, s), "H2D staged xfixed")) return -1;
     if (bqb_check(hipMemcpyAsync(c->p.ld_sx, c->ld_sx, sizeof(int) * B * c->nload, hipMemcpyHostToDevice, s), "H2D staged solX")) return -1;
+    /* inherited cut sets go straight into their slots' cut arrays */
+    if (c->wstage)
+        for (int i = 0; i < c->hi; ++i) {
+            const struct cut_pack *k = c->wstage[i];
+            if (!k) continue;
+            const size_t ti = (size_t)i * c->d.max_tri, pi = (size_t)i * c->d.max_pent, hi = (size_t)i * c->d.max_hept;
+            if ((k->ntri && (bqb_check(hipMemcpyAsync(c->p.tri + ti, k->tri, sizeof(TriCut) * k->ntri, hipMemcpyHostToDevice, s), "H2D cuts") ||
+                             bqb_check(hipMemcpyAsync(c->p.tri_y + ti, k->y, sizeof(double) * k->ntri, hipMemcpyHostToDevice, s), "H2D cuts"))) ||
+                (k->npent && (bqb_check(hipMemcpyAsync(c->p.pent + pi, k->pent, sizeof(KgonCut) * k->npent, hipMemcpyHostToDevice, s), "H2D cuts") ||
+                              bqb_check(hipMemcpyAsync(c->p.pent_y + pi, k->y + k->ntri, sizeof(double) * k->npent, hipMemcpyHostToDevice, s), "H2D cuts"))) ||
+                (k->nhept && (bqb_check(hipMemcpyAsync(c->p.hept + hi, k->hept, sizeof(KgonCut) * k->nhept, hipMemcpyHostToDevice, s), "H2D cuts") ||
+                              bqb_check(hipMemcpyAsync(c->p.hept_y + hi, k->y + k->ntri + k->npent, sizeof(double) * k->nhept, hipMemcpyHostToDevice, s), "H2D cuts"))))
+                return -1;
+        }
     if (bqb_ctx_run(c, PH_LOAD, 0)) return -1;
     for (int i = 0; i < c->hi; ++i) c->of_slot[i] = -1;
     c->nload = 0;
     /* the host copies of the staging must stay valid until the copies ran */
-    return ctx_sync(c);
+    if (ctx_sync(c)) return -1;
+    if (c->wstage)
+        for (int i = 0; i < c->hi; ++i) { free(c->wstage[i]); c->wstage[i] = NULL; }
+    return 0;
+}
+
+int bqb_ctx_set_warm(bqb_ctx *c, int on)
+{
+    if (bqb_ctx_busy(c) || c->nload) { bqb_set_error("set_warm: nodes in flight"); return -1; }
+    if (on && !c->wstage) {
+        c->wstage = calloc(c->max_nodes, sizeof(struct cut_pack *));
+        if (!c->wstage) { bqb_set_error("calloc"); return -1; }
+    }
+    graphs_clear(c);           /* the basic train changes */
+    c->warm = on ? 1 : 0;
+    return 0;
+}
+
+int bqb_ctx_push_cuts(bqb_ctx *c, int slot, int ntri, const int *tri, int npent, const int *pent, int nhept,
+                      const int *hept, const double *gamma)
+{
+    if (!c->warm) { bqb_set_error("push_cuts: inherited cuts are off (bqb_ctx_set_warm)"); return -1; }
+    if (slot < 0 || slot >= c->max_nodes || c->of_slot[slot] < 0) {
+        bqb_set_error("push_cuts: slot %d holds no node pushed since the last advance", slot);
+        return -1;
+    }
+    if (ntri < 0 || npent < 0 || nhept < 0 || ntri > c->d.max_tri || npent > c->d.max_pent || nhept > c->d.max_hept) {
+        bqb_set_error("push_cuts: %d/%d/%d cuts exceed the capacities %d/%d/%d", ntri, npent, nhept, c->d.max_tri,
+                      c->d.max_pent, c->d.max_hept);
+        return -1;
+    }
+    NodeState *st = &c->ld_st[c->of_slot[slot]];
+    const int B = c->d.n0 - 1;
+    const int *xf = c->ld_xf + (size_t)c->of_slot[slot] * B;
+    int n = 1;
+    for (int i = 0; i < B; ++i) n += !xf[i];
+    /* records must be valid cuts of the node's n x n subproblem: the
+     * separations' vertex order (triangles i > j > k as
+     * cutting_planes.c:58-62, k-gonal vertices distinct) and type ranges */
+    for (int q = 0; q < ntri; ++q) {
+        const int *r = tri + 4 * q;
+        if (!(r[0] < n && r[0] > r[1] && r[1] > r[2] && r[2] >= 0 && r[3] >= 1 && r[3] <= 4)) {
+            bqb_set_error("push_cuts: triangle cut %d (%d %d %d type %d) is not a cut of an %d-vertex subproblem",
+                          q, r[0], r[1], r[2], r[3], n);
+            return -1;
+        }
+    }
+    for (int q = 0; q < npent + nhept; ++q) {
+        const int k = q < npent ? 5 : 7, *r = q < npent ? pent + 8 * q : hept + 8 * (q - npent);
+        int ok = r[0] >= 1 && r[0] <= (k == 5 ? 3 : 4);
+        for (int a = 0; a < k && ok; ++a) {
+            ok = r[1 + a] >= 0 && r[1 + a] < n;
+            for (int b = 0; b < a && ok; ++b) ok = r[1 + a] != r[1 + b];
+        }
+        if (!ok) { bqb_set_error("push_cuts: %d-gonal cut %d is not a cut of an %d-vertex subproblem", k, q, n); return -1; }
+    }
+    for (int q = 0; q < ntri + npent + nhept; ++q)
+        if (!(gamma[q] >= 0.0) || !isfinite(gamma[q])) {
+            bqb_set_error("push_cuts: multiplier %d is not a finite value >= 0", q);
+            return -1;
+        }
+    struct cut_pack *k = pack_alloc(ntri, npent, nhept);
+    if (!k) return -1;
+    memcpy(k->tri, tri, sizeof(TriCut) * ntri);
+    memcpy(k->pent, pent, sizeof(KgonCut) * npent);
+    memcpy(k->hept, hept, sizeof(KgonCut) * nhept);
+    memcpy(k->y, gamma, sizeof(double) * (ntri + npent + nhept));
+    free(c->wstage[slot]);
+    c->wstage[slot] = k;
+    st->warm = ntri + npent + nhept > 0;
+    st->NIneq = ntri;
+    st->NPentIneq = npent;
+    st->NHeptaIneq = nhept;
+    return 0;
+}
+
+int bqb_ctx_last_cuts(const bqb_ctx *c, int *ntri, const int **tri, int *npent, const int **pent, int *nhept,
+                      const int **hept, const double **gamma)
+{
+    const struct cut_pack *k = c->pop_cuts;
+    if (ntri) *ntri = k ? k->ntri : 0;
+    if (npent) *npent = k ? k->npent : 0;
+    if (nhept) *nhept = k ? k->nhept : 0;
+    if (tri) *tri = k ? (const int *)k->tri : NULL;
+    if (pent) *pent = k ? (const int *)k->pent : NULL;
+    if (hept) *hept = k ? (const int *)k->hept : NULL;
+    if (gamma) *gamma = k ? k->y : NULL;
+    return 0;
 }
 
 static int fetch_states(bqb_ctx *c)
@@ -594,7 +746,8 @@ static int fetch_states(bqb_ctx *c)
     return ctx_sync(c);
 }
 
-static int done_push(bqb_ctx *c, long long tag, const NodeState *st, const double *frac, const int *xb)
+static int done_push(bqb_ctx *c, long long tag, const NodeState *st, const double *frac, const int *xb,
+                     struct cut_pack *cuts)
 {
     const int B = c->d.n0 - 1;
     if (c->ndone == c->done_cap) {
@@ -605,6 +758,7 @@ static int done_push(bqb_ctx *c, long long tag, const NodeState *st, const doubl
         for (int i = c->ndone; i < cap; ++i) {
             d[i].frac = malloc(sizeof(double) * B);
             d[i].xb = malloc(sizeof(int) * B);
+            d[i].cuts = NULL;
             if (!d[i].frac || !d[i].xb) {
                 for (int j = c->ndone; j <= i; ++j) { free(d[j].frac); free(d[j].xb); }
                 free(d);
@@ -622,6 +776,7 @@ static int done_push(bqb_ctx *c, long long tag, const NodeState *st, const doubl
     e->st = *st;
     memcpy(e->frac, frac, sizeof(double) * B);
     memcpy(e->xb, xb, sizeof(int) * B);
+    e->cuts = cuts;
     ++c->ndone;
     return 0;
 }
@@ -706,12 +861,46 @@ static int collect(bqb_ctx *c)
     if (c->capture)
         for (int i = 0; i < c->hi; ++i)
             if (c->tag[i] >= 0 && !c->h_st[i].active && traj_start(c, i)) return -1;
-    if (ctx_sync(c)) return -1;
+    /* warm contexts: the final cut set and multipliers of every node that
+     * may be branched (not pruned, no error) */
+    struct cut_pack **fin = NULL;
+    if (c->warm) {
+        fin = calloc(c->hi, sizeof *fin);
+        if (!fin) { bqb_set_error("calloc"); return -1; }
+        for (int i = 0; i < c->hi; ++i) {
+            const NodeState *t = &c->h_st[i];
+            const int m = t->NIneq + t->NPentIneq + t->NHeptaIneq;
+            if (c->tag[i] < 0 || t->active || t->prune || t->status || m == 0) continue;
+            struct cut_pack *k = fin[i] = pack_alloc(t->NIneq, t->NPentIneq, t->NHeptaIneq);
+            hipStream_t st = c->stream;
+            if (!k ||
+                (k->ntri && bqb_check(hipMemcpyAsync(k->tri, c->p.tri + (size_t)i * c->d.max_tri, sizeof(TriCut) * k->ntri, hipMemcpyDeviceToHost, st), "D2H cuts")) ||
+                (k->npent && bqb_check(hipMemcpyAsync(k->pent, c->p.pent + (size_t)i * c->d.max_pent, sizeof(KgonCut) * k->npent, hipMemcpyDeviceToHost, st), "D2H cuts")) ||
+                (k->nhept && bqb_check(hipMemcpyAsync(k->hept, c->p.hept + (size_t)i * c->d.max_hept, sizeof(KgonCut) * k->nhept, hipMemcpyDeviceToHost, st), "D2H cuts")) ||
+                bqb_check(hipMemcpyAsync(k->y, c->p.gamma + (size_t)i * c->d.max_m, sizeof(double) * m, hipMemcpyDeviceToHost, st), "D2H gamma")) {
+                ctx_sync(c);
+                for (int j = 0; j <= i; ++j) free(fin[j]);
+                free(fin);
+                return -1;
+            }
+        }
+    }
+    if (ctx_sync(c)) {
+        if (fin) for (int i = 0; i < c->hi; ++i) free(fin[i]);
+        free(fin);
+        return -1;
+    }
     for (int i = 0; i < c->hi; ++i) {
         if (c->tag[i] < 0 || c->h_st[i].active) continue;
-        if (done_push(c, c->tag[i], &c->h_st[i], c->h_frac + i * B, c->h_xb + i * B)) return -1;
+        if (done_push(c, c->tag[i], &c->h_st[i], c->h_frac + i * B, c->h_xb + i * B, fin ? fin[i] : NULL)) {
+            for (int j = i; j < c->hi; ++j) free(fin ? fin[j] : NULL);
+            free(fin);
+            return -1;
+        }
+        if (fin) fin[i] = NULL;
         c->tag[i] = -1;
     }
+    free(fin);
     /* free slots, lowest index on top of the stack */
     c->nfree = 0;
     for (int i = c->max_nodes - 1; i >= 0; --i)
@@ -735,6 +924,13 @@ static int basic_train(bqb_ctx *c)
     RUN(112, 0);             /* learned warm start / ranking of the new cuts */
     RUN(PH_BUNDLE_INIT, 0);
     RUN(106, 0);             /* op_Bt gather lists */
+    if (c->warm) {           /* inherited multipliers: f(gamma) as the first center */
+        RUN(PH_WARM, 0);
+        RUN(107, 0);         /* L0 = L - B'(gamma) */
+        RUN(101, 0);         /* IPM: L0 -> X_test */
+        RUN(PH_REBUILD, 1);
+        if (c->d.certify) RUN(114, 0);   /* certify the warm-started center */
+    }
     return 0;
 }
 
@@ -951,6 +1147,9 @@ int bqb_ctx_pop(bqb_ctx *c, long long *tag, double *bound, double *fracsol, int 
         stats[4] = s->ipm_iters; stats[5] = s->max_cuts; stats[6] = s->max_bundle_seen; stats[7] = s->n;
         stats[8] = s->flags;
     }
+    free(c->pop_cuts);
+    c->pop_cuts = e->cuts;
+    c->done[c->done_head].cuts = NULL;
     c->done_head = (c->done_head + 1) % c->done_cap;
     --c->ndone;
     return 1;

@@ -18,9 +18,11 @@ long because its start (every slot beginning a node at once) and its end
 whatever its length; a B&B node queue that never runs empty does not pay
 them (16384 nodes per step give 4540 bounds/s, 32768 6040, 65536 6750).  Inputs are uploaded
 inside the call (a few hundred KB; the timed region includes it).
-value = node bounds per second.  For comparison, config reports the first
-2048 nodes streamed through a single group and the first 512 bounded as one
-fixed batch (every launch waits for the slowest node).  The roofline object covers the dominant
+value = node bounds per second.  For comparison, config reports the nodes
+that went past the basic bound (the whole SDPbound pipeline) streamed
+through the same groups, the step streamed through one group of 8192 slots,
+and the first 512 bounded as one fixed batch (every launch waits for the
+slowest node).  The roofline object covers the dominant
 kernel, the batched IPM: algorithmic FP64 flops (6 n^3 per iteration + n^3/3 per line-search
 Cholesky test, counted on the device) over the summed duration of its launches
 (HIP event pairs on the launch stream, during the timed steps); with twelve
@@ -109,6 +111,29 @@ PROOF_FORMS = (
     ("batched_inherited_mixed", dict(batch=64, warm=True, slices=4)),
     ("batched_inherited_mixed_learned", dict(batch=64, warm=True, slices=4, learned=True)),
 )
+
+
+def proof_one(api, I, seed, form, reps):
+    """One time-to-proof instance (G(60, 0.5), unit weights, triangles only)
+    in one form, after one untimed proof, `reps` times."""
+    P = api.make_params(include_Pent=0, include_Hepta=0)
+    kw = dict(dict(PROOF_FORMS)[form])
+    model = None
+    if kw.pop("learned", False):
+        from parallel_biqbin_maxcut_amd import learn
+        model = learn.MLP(params=np.load(os.path.join(ROOT, "parallel_biqbin_maxcut_amd", "models",
+                                                      "cut_model.npz"), allow_pickle=False)["params"])
+        kw["model"] = model
+    L = api.maxcut_L(I.gnp_adj(60, 0.5, seed, weights="unit"))
+    api.solve(L, params=P, seed=2020, **kw)
+    secs = []
+    for _ in range(reps):
+        st, _ = api.solve(L, params=P, seed=2020, **kw)
+        secs.append(round(st.seconds, 4))
+    if model is not None:
+        model.close()
+    return {"mode": "proof_one", "seed": seed, "form": form, "optimum": st.best, "nodes": int(st.nodes),
+            "batches": int(st.batches), "seconds": secs}
 
 
 def time_to_proof(api, I, seeds, cpu_seeds):
@@ -379,6 +404,9 @@ def main():
     ap.add_argument("--ipm-workload", default="cfg2", choices=("cfg2", "cfg3"),
                     help="instance of --ipm-isolated: cfg2 G(100, 0.3) or cfg3 G(250, 0.1)")
     ap.add_argument("--ipm-slices", type=int, default=0, help="--ipm-isolated products: 0 FP64, 2..4 bf16 slices")
+    ap.add_argument("--proof-seed", type=int, default=0,
+                    help="time the proof of one G(60, 0.5) instance (seed) in --proof-form, --steps times (profiling)")
+    ap.add_argument("--proof-form", default="plain", choices=[nm for nm, _ in PROOF_FORMS])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: bounding runs on one GPU; --gpus {args.gpus} is not supported "
@@ -390,6 +418,9 @@ def main():
     if args.ipm_isolated:
         print(json.dumps(ipm_isolated(api, I, args.seed, args.nodes, max(1, args.steps), args.ipm_workload,
                                       args.ipm_slices)), flush=True)
+        return
+    if args.proof_seed:
+        print(json.dumps(proof_one(api, I, args.proof_seed, args.proof_form, max(1, args.steps))), flush=True)
         return
 
     SP, P, xf, sx, seeds = build_workload(api, I, args.seed, args.per_step)
@@ -432,21 +463,31 @@ def main():
     if np.any(r["status"]):
         raise SystemExit(f"node bound failures: {np.unique(r['status'], return_counts=True)}")
 
-    # comparisons: the first 2048 nodes through one slot group, the first 512
-    # as one fixed batch (all launches wait for its slowest node)
-    single_batch = single_group = None
+    # comparisons: the nodes that went past the basic bound (the whole
+    # SDPbound pipeline) through the same groups; the step through one group
+    # of 8192 slots; the first 512 nodes as one fixed batch (every launch
+    # waits for its slowest node).  All must give the step's bounds.
+    single_batch = one_group = full_pipeline = None
+    deep = np.nonzero(r["stats"][:, 2] > 0)[0]
     if not args.no_single:
-        one = api.BoundContext(SP, params=P, max_nodes=args.nodes)
-        k = min(2048, args.per_step)
         tb = time.perf_counter()
-        rg = one.bound(xf[:k], sx[:k], LB=LB, diff=diff, is_root=0, seeds=seeds[:k], stream=True)
-        single_group = round(k / (time.perf_counter() - tb), 2)
+        rd = ctx.bound(xf[deep], sx[deep], LB=LB, diff=diff, is_root=0, seeds=seeds[deep], stream=True)
+        full_pipeline = round(len(deep) / (time.perf_counter() - tb), 2)
+        ctx.close()
+        one = api.BoundContext(SP, params=P, max_nodes=8192)
+        tb = time.perf_counter()
+        ro = one.bound(xf, sx, LB=LB, diff=diff, is_root=0, seeds=seeds, stream=True)
+        one_group = round(args.per_step / (time.perf_counter() - tb), 2)
         tb = time.perf_counter()
         rb = one.bound(xf[: args.nodes], sx[: args.nodes], LB=LB, diff=diff, is_root=0, seeds=seeds[: args.nodes])
         single_batch = round(args.nodes / (time.perf_counter() - tb), 2)
         one.close()
-        if not (np.array_equal(rb["bound"], r["bound"][: args.nodes]) and np.array_equal(rg["bound"], r["bound"][:k])):
+        if not (np.array_equal(rb["bound"], r["bound"][: args.nodes]) and np.array_equal(ro["bound"], r["bound"])
+                and np.array_equal(rd["bound"], r["bound"][deep])):
             raise SystemExit("streamed and batched bounds differ")
+        progress("comparisons done")
+    else:
+        ctx.close()
 
     ms_per_step = dt / args.steps * 1e3
     value = args.per_step * args.steps / dt
@@ -472,8 +513,10 @@ def main():
             "nodes_per_launch": args.nodes,
             "slot_groups": args.groups,
             "nodes_per_step": args.per_step,
-            "one_group_2048_nodes_bounds_per_s": single_group,
+            "one_group_8192_slots_bounds_per_s": one_group,
             "single_batch_bounds_per_s": single_batch,
+            "full_pipeline_nodes": int(len(deep)),
+            "full_pipeline_bounds_per_s": full_pipeline,
             "seed": args.seed,
             "incumbent": LB,
             "root_gap": round(diff, 4),

@@ -200,6 +200,17 @@ def time_to_proof(api, I, seeds, cpu_seeds):
                                  f"GPU {out['optimum'][i]}")
         out["cpu_oracle_seconds_plain"] = cs
         out["cpu_oracle_cores"] = 1
+        # the other instances' optima against the CPU oracle (untimed, one
+        # sequential B&B per thread)
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(min(16, len(os.sched_getaffinity(0)))) as ex:
+            rest = list(ex.map(lambda L: O.bab(L, O.default_params(**d), seed=2020, batch=1)[0].best,
+                               Ls[cpu_seeds:]))
+        for i, b in enumerate(rest, start=cpu_seeds):
+            if b != out["optimum"][i]:
+                raise SystemExit(f"time-to-proof instance {seeds[i]}: CPU oracle optimum {b}, GPU {out['optimum'][i]}")
+        out["optima_checked_against_cpu_oracle"] = len(Ls)
+        progress("time-to-proof optima checked against the CPU oracle")
     return out
 
 
@@ -565,6 +576,11 @@ def main():
         rel = np.abs(r["bound"][:sample][ok] - cbounds[ok]) / np.abs(cbounds[ok])
         line["cpu_baseline"]["max_rel_bound_diff_vs_gpu"] = float(rel.max()) if rel.size else None
         line["cpu_baseline"]["port_over_reference_root_time"] = PORT_OVER_REFERENCE
+        # the reference binary's throughput on the same cores, estimated from
+        # the port's through the measured per-node time ratio (cfg2 root)
+        line["cpu_baseline"]["reference_estimate_bounds_per_s"] = round(
+            cb["value"] * PORT_OVER_REFERENCE["cfg2 G(100, 0.3) root"], 1)
+        progress("CPU baseline done")
     print(json.dumps(line), flush=True)
 
 

@@ -21,8 +21,9 @@ inside the call (a few hundred KB; the timed region includes it).
 value = node bounds per second.  For comparison, config reports the nodes
 that went past the basic bound (the whole SDPbound pipeline) streamed
 through the same groups, the step streamed through one group of 8192 slots,
-and the first 512 bounded as one fixed batch (every launch waits for the
-slowest node).  The roofline object covers the dominant
+the first 512 bounded as one fixed batch (every launch waits for the
+slowest node), and the first 64 bounded one at a time (the plain GPU form).
+The roofline object covers the dominant
 kernel, the batched IPM: algorithmic FP64 flops (6 n^3 per iteration + n^3/3 per line-search
 Cholesky test, counted on the device) over the summed duration of its launches
 (HIP event pairs on the launch stream, during the timed steps); with twelve
@@ -478,7 +479,7 @@ def main():
     # SDPbound pipeline) through the same groups; the step through one group
     # of 8192 slots; the first 512 nodes as one fixed batch (every launch
     # waits for its slowest node).  All must give the step's bounds.
-    single_batch = one_group = full_pipeline = None
+    single_batch = one_group = full_pipeline = plain_one = None
     deep = np.nonzero(r["stats"][:, 2] > 0)[0]
     if not args.no_single:
         tb = time.perf_counter()
@@ -493,8 +494,16 @@ def main():
         rb = one.bound(xf[: args.nodes], sx[: args.nodes], LB=LB, diff=diff, is_root=0, seeds=seeds[: args.nodes])
         single_batch = round(args.nodes / (time.perf_counter() - tb), 2)
         one.close()
+        # the plain GPU form: one node at a time (a one-slot context), on the
+        # first 64 nodes of the step
+        solo = api.BoundContext(SP, params=P, max_nodes=1)
+        k1 = min(64, args.per_step)
+        tb = time.perf_counter()
+        r1 = solo.bound(xf[:k1], sx[:k1], LB=LB, diff=diff, is_root=0, seeds=seeds[:k1], stream=True)
+        plain_one = round(k1 / (time.perf_counter() - tb), 2)
+        solo.close()
         if not (np.array_equal(rb["bound"], r["bound"][: args.nodes]) and np.array_equal(ro["bound"], r["bound"])
-                and np.array_equal(rd["bound"], r["bound"][deep])):
+                and np.array_equal(rd["bound"], r["bound"][deep]) and np.array_equal(r1["bound"], r["bound"][:k1])):
             raise SystemExit("streamed and batched bounds differ")
         progress("comparisons done")
     else:
@@ -528,6 +537,10 @@ def main():
             "single_batch_bounds_per_s": single_batch,
             "full_pipeline_nodes": int(len(deep)),
             "full_pipeline_bounds_per_s": full_pipeline,
+            # node bounds/s of the plain GPU form (one node at a time) on the
+            # same nodes, and the batched form's speedup over it
+            "plain_one_at_a_time_bounds_per_s": plain_one,
+            "speedup_batched_vs_plain_bounds_per_s": round(value / plain_one, 1) if plain_one else None,
             "seed": args.seed,
             "incumbent": LB,
             "root_gap": round(diff, 4),

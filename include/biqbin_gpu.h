@@ -369,13 +369,14 @@ int bqb_ctx_trajectory(const bqb_ctx *c, int i, bqb_trajectory *out);
 void bqb_ctx_trajectory_clear(bqb_ctx *c);
 
 /* Stepwise use of a context (tests and diagnostics): load a batch into
- * slots 0..nodes-1, run one phase (0-15 control phases; 100 basic IPM,
+ * slots 0..nodes-1, run one phase (0-16 control phases; 100 basic IPM,
  * 101 IPM on L0, 102 heuristic,
  * 103 triangle update, 104/105 pentagonal/heptagonal update, 106 op_Bt
  * gather lists, 107 L0 = L - B'(gamma or gamma_test if arg), 108 bundle QP
  * (arg: an upper bound on every bundle size, 0 if unknown; a bound <= 16
  * selects the small-LDS QP kernel), 109 bundle step, 110/111 certificate of
- * the basic bound / of a new bundle center, 112/113 learned model on the
+ * the basic bound / of the bundle center of the nodes whose center moved
+ * since their last certificate (mask phase 16), 112/113 learned model on the
  * cuts of the first / of a round's separation, 114 certificate of a
  * warm-started center, 115 upload the nodes pushed since the last advance
  * and run only their basic sequence), run the whole bound,
@@ -407,7 +408,8 @@ int bqb_ctx_set_heuristic(bqb_ctx *c, bqb_heuristic_fn fn, void *user);
  * reference's arithmetic), 2..4 on the bf16 matrix cores as in
  * bqb_ipm_batch_mixed.  certify = 1 (implied by slices > 0) makes every
  * value the node prunes on and returns a certified upper bound: the basic
- * bound and every new bundle center (a serious step, bundle.c:93-135) get a
+ * bound, and at the end of every outer round the bundle center if it moved
+ * (serious steps, bundle.c:93-135) since its last certificate, get a
  * certificate  sum(gamma) + e'y + n * bound(-lambda_min(Diag(y) - L + B'gamma))
  * computed with directed rounding (src/bounding.c:109, 180, 197 prune on it). */
 int bqb_ctx_set_precision(bqb_ctx *c, int slices, int certify);

@@ -215,7 +215,7 @@ class NodeState(C.Structure):
         "n", "status", "active", "prune", "giveup", "done", "count", "bdl_iter", "NIneq",
         "NPentIneq", "NHeptaIneq", "bundle", "sep_tri", "sep_pent", "sep_hept", "heur_idx",
         "improved", "ipm_calls", "ipm_iters", "max_cuts", "max_bundle_seen", "is_root", "serious",
-        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags", "warm")] + [(f, C.c_double) for f in (
+        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags", "warm", "cert_stale")] + [(f, C.c_double) for f in (
         "fixedvalue", "f", "f_test", "t", "bound", "basic_bound", "oldf", "LB", "diff", "viol3",
         "viol5", "gap", "del_", "lmax", "f_cert")] + [("seed", C.c_uint64)]
 
@@ -223,10 +223,10 @@ class NodeState(C.Structure):
 BUF = dict(state=0, L=1, L0=2, X=3, Xt=4, tri=5, tri_y=6, pent=7, pent_y=8, hept=9, hept_y=10,
            gamma=11, gtest=12, g=13, G=14, F=15, lam=16, tri_list=17, tri_lval=18, kg_list=19,
            kg_lval=20, fracsol=21, xbest=22, xheur=23, mask=24, phi=25, Xb=26, nvec=27, dgamma=28,
-           sa_val=29)
+           sa_val=29, ycen=30)
 PH = dict(CERT_BASIC=110, CERT_CENTER=111, SETUP=0, POST_BASIC=1, CHECK_BASIC=2, BUNDLE_INIT=3, OUTER_BEGIN=4, OUTER_A=5, OUTER_B=6,
           SEP_PENT=7, SEP_HEPT=8, OUTER_C=9, REBUILD=10, FINISH=11, FCT_FINISH=12, LOAD=13,
-          BUNDLE_NEXT=14, WARM=15, IPM_BASIC=100, IPM_L0=101, BASIC_TRAIN=115,
+          BUNDLE_NEXT=14, WARM=15, CERT_MASK=16, IPM_BASIC=100, IPM_L0=101, BASIC_TRAIN=115,
           HEUR=102, TRISEP=103, PENT=104, HEPT=105, CSR=106, OPBT=107, QP=108, STEP=109)
 
 

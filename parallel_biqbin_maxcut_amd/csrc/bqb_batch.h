@@ -46,6 +46,8 @@ typedef struct NodeState {
     int flags;          /* BQB_FLAG_* events of the node (reported in stats[8]) */
     int warm;           /* loaded with inherited cuts and multipliers: its first
                            bundle center is evaluated at them (until done) */
+    int cert_stale;     /* the center moved (serious step) since f_cert was
+                           computed; its y is in ycen */
     double fixedvalue, f, f_test, t, bound, basic_bound, oldf;
     double LB, diff, viol3, viol5, gap, del, lmax;
     double f_cert;      /* certified upper bound of f (when certifying) */
@@ -92,6 +94,7 @@ typedef struct BatchPtrs {
     double *ipm_work;               /* [nodes*3*ld*ld] */
     double *phi;                    /* [nodes] */
     double *ytest;                  /* [nodes*ld] dual vector y of the last IPM */
+    double *ycen;                   /* [nodes*ld] y of the bundle center's evaluation (certification) */
     int *ipm_iters, *ipm_trials, *ipm_status;   /* [nodes] */
     double *Xb;                     /* [nodes*max_bundle*ld*ld] bundle matrices */
     double *G;                      /* [nodes*max_bundle*max_m], column i at i*m (ld = m) */

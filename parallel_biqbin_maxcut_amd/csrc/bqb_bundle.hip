@@ -561,6 +561,12 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
     if (serious) {
         for (int l = threadIdx.x; l < m; l += BT) gam[l] = gt[l];
         f = f_test;
+        // the new center's y, for its certificate at the round end
+        if (d.certify) {
+            const double *yt = p.ytest + (size_t)node * d.ld;
+            double *yc = p.ycen + (size_t)node * d.ld;
+            for (int i = threadIdx.x; i < s.n; i += BT) yc[i] = yt[i];
+        }
         // X = X_bundle * lambda, each entry summed over the bundle in order
         double *X = p.X + node * slab;
         const double *Xb = p.Xb + (size_t)node * d.max_bundle * slab;
@@ -621,6 +627,7 @@ __global__ __launch_bounds__(BT) void bundle_step_kernel(BatchDims d, BatchPtrs 
         sp->del = del;
         sp->lmax = lmax;
         sp->serious = serious;
+        if (serious && d.certify) sp->cert_stale = 1;
         sp->ipm_calls = s.ipm_calls + 1;
         sp->ipm_iters = s.ipm_iters + p.ipm_iters[node];
         sp->bdl_left = s.bdl_left - 1;

@@ -51,7 +51,10 @@ __device__ __forceinline__ void l0_entry(const CutSource *cs, double L, int i, i
     for (int q = cs->ptr[pid]; q < cs->ptr[pid + 1]; ++q) {
         const int w = cs->lst[q];
         const int cut = w >> 1;
-        const double t = cs->gam[cut];
+        // the bundle's multipliers can carry rounding-level negatives
+        // (gamma + t (eta - G lambda) cancels where eta > 0); weak duality
+        // needs gamma >= 0, so the certificate is that of max(gamma, 0)
+        const double t = fmax(cs->gam[cut], 0.0);
         double t_rn, t_lo, t_hi;
         if (cut < cs->ntri) { t_rn = t_lo = t_hi = 0.5 * t; }
         else if (cut < cs->ntp) { t_rn = t_lo = t_hi = 0.25 * t; }
@@ -243,8 +246,12 @@ __global__ __launch_bounds__(CE) void cert_ipm_kernel(const int *nvec, const dou
 
 // ---- the node bound: certificate of the point just evaluated ----
 // mode 0: basic bound (no cuts; phi of the IPM on L, y in ytest);
-// mode 1: after a bundle step that was serious (gamma = gamma_test, the cut
-// lists and gather lists of the node, y of the IPM on L - B'gamma_test);
+// mode 1: the center of a node whose center moved (serious steps) since its
+// last certificate, at the end of its round (gamma the center, y of the
+// center's evaluation saved in ycen; the cut lists and gather lists are
+// those of the bundle iterations that moved it), the value the round-end
+// prune test reads -- the same value a certificate after every serious step
+// would leave there;
 // mode 2: after the rebuild of a round that made the warm-started center
 // gamma the new f (y of the IPM on L - B'gamma).
 // The certified value replaces NodeState.f_cert.
@@ -257,19 +264,19 @@ __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p,
     NodeState *sp = p.st + node;
     const NodeState s = *sp;
     if (!s.active || s.status) return;
-    if (mode == 1 && !s.serious) return;
+    if (mode == 1 && !s.cert_stale) return;
     if (mode == 2 && (s.stage != BQB_STAGE_LOOP || s.rphase != 0)) return;
     const int n = s.n;
     if (n < 1 || n > d.ld) return;
     const size_t slab = (size_t)d.ld * d.ld;
     double *Zt = p.ipm_work + (size_t)node * d.work_slabs * slab;
-    const double *y = p.ytest + (size_t)node * d.ld;
+    const double *y = (mode == 1 ? p.ycen : p.ytest) + (size_t)node * d.ld;
     CutSource cs;
     const CutSource *csp = nullptr;
     double sg = 0.0;
     if (mode >= 1) {
         const int m = s.NIneq + s.NPentIneq + s.NHeptaIneq;
-        const double *gam = (mode == 1 ? p.gtest : p.gamma) + (size_t)node * d.max_m;
+        const double *gam = p.gamma + (size_t)node * d.max_m;
         cs.ptr = p.csr_ptr + (size_t)node * (d.npairs + 1);
         cs.lst = p.csr_cut + (size_t)node * d.max_contrib;
         cs.gam = gam;
@@ -277,11 +284,11 @@ __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p,
         cs.ntp = s.NIneq + s.NPentIneq;
         csp = &cs;
         double part = 0.0;
-        for (int c = threadIdx.x; c < m; c += CE) part = bqb_add_ru(part, gam[c]);
+        for (int c = threadIdx.x; c < m; c += CE) part = bqb_add_ru(part, fmax(gam[c], 0.0));
         sg = red_ru(part, red);
     }
     const double v = certify(n, d.ld, p.L + node * slab, y, csp, sg, Zt, Zt + slab, rs, red, &s_fail, &s_piv);
-    if (threadIdx.x == 0) sp->f_cert = v;
+    if (threadIdx.x == 0) { sp->f_cert = v; sp->cert_stale = 0; }
 }
 
 }  // namespace bqb

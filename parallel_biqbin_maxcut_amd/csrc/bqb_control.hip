@@ -197,6 +197,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             sp->max_cuts = sp->max_bundle_seen = 0;
             sp->viol3 = sp->viol5 = 0.0;
             sp->f_cert = INFINITY;       // until a certificate is computed
+            sp->cert_stale = 0;
             sp->cuts_changed = 1;
             p.nvec[node] = n;
             p.mask[node] = 1;
@@ -214,7 +215,8 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         switch (phase) {
         case PH_POST_BASIC: case PH_CHECK_BASIC: case PH_BUNDLE_INIT: want = BQB_STAGE_NEW; break;
         case PH_OUTER_BEGIN: case PH_OUTER_A: case PH_OUTER_B: case PH_SEP_PENT: case PH_SEP_HEPT:
-        case PH_OUTER_C: case PH_REBUILD: case PH_BUNDLE_NEXT: case PH_WARM: want = BQB_STAGE_LOOP; break;
+        case PH_OUTER_C: case PH_REBUILD: case PH_BUNDLE_NEXT: case PH_WARM: case PH_CERT_MASK:
+            want = BQB_STAGE_LOOP; break;
         default: break;
         }
         // outer rounds advance per node: begin (rphase 0), bundle
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         case PH_OUTER_BEGIN: want_r = 0; break;
         case PH_BUNDLE_NEXT: want_r = 1; break;
         case PH_OUTER_A: case PH_OUTER_B: case PH_SEP_PENT: case PH_SEP_HEPT: case PH_OUTER_C:
-        case PH_REBUILD: case PH_WARM: want_r = 2; break;
+        case PH_REBUILD: case PH_WARM: case PH_CERT_MASK: want_r = 2; break;
         default: break;
         }
         if ((want >= 0 && s.stage != want) || (want_r >= 0 && s.rphase != want_r)) {
@@ -240,6 +242,11 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
 
     if (phase == PH_WARM) {
         if (threadIdx.x == 0) p.mask[node] = (s.active && s.warm) ? 1 : 0;
+        return;
+    }
+
+    if (phase == PH_CERT_MASK) {
+        if (threadIdx.x == 0) p.mask[node] = (s.active && s.cert_stale) ? 1 : 0;
         return;
     }
 

@@ -17,6 +17,7 @@ enum {
     PH_FCT_FINISH = 12,  /* fct_eval tail: f = phi + sum(gamma), g = 1 - B(X_test) */
     PH_LOAD = 13,        /* copy staged nodes into their slots, mask = newly loaded */
     PH_BUNDLE_NEXT = 14, /* mask for bundle iteration arg: active LOOP nodes with arg < bdl_iter */
-    PH_WARM = 15         /* mask: nodes whose inherited multipliers are evaluated next */
+    PH_WARM = 15,        /* mask: nodes whose inherited multipliers are evaluated next */
+    PH_CERT_MASK = 16    /* mask: nodes at a round end whose center moved since its certificate */
 };
 #endif

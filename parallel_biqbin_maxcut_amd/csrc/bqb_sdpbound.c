@@ -306,6 +306,7 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         {(void **)&c->p.ipm_work, sizeof(double) * N * BQB_IPM_WORK_SLABS(1) * slab},
         {(void **)&c->p.phi, sizeof(double) * N},
         {(void **)&c->p.ytest, sizeof(double) * N * d->ld},
+        {(void **)&c->p.ycen, sizeof(double) * N * d->ld},
         {(void **)&c->p.ipm_iters, sizeof(int) * N},
         {(void **)&c->p.ipm_trials, sizeof(int) * N},
         {(void **)&c->p.ipm_status, sizeof(int) * N},
@@ -572,7 +573,7 @@ int bqb_ctx_run(bqb_ctx *c, int what, int arg)
                                           c->d_rec_feat, c->d_rec_key, c->d_lidx, c->d_lkeep, what == 113, s),
                          "learn launch");
     default:
-        if (what >= 0 && what <= PH_WARM)
+        if (what >= 0 && what <= PH_CERT_MASK)
             return bqb_check(bqb_launch_control(d, p, what, arg, s), "control launch");
         bqb_set_error("unknown phase %d", what);
         return -1;
@@ -948,7 +949,12 @@ static int round_train(bqb_ctx *c, int nb, int kmax)
         RUN(107, 1);         /* L0 = L - B'(gamma_test) */
         RUN(101, 0);         /* IPM: L0 -> X_test */
         RUN(109, 0);         /* serious / null step */
-        if (c->d.certify) RUN(111, 0);   /* certify the new center */
+    }
+    /* the certificate of the center the round ends with, where it moved:
+     * the value the round-end prune test reads (bounding.c:180, 197) */
+    if (c->d.certify) {
+        RUN(PH_CERT_MASK, 0);
+        RUN(111, 0);
     }
     RUN(PH_OUTER_A, 0);
     RUN(102, 0);             /* heuristic */
@@ -1506,6 +1512,7 @@ static int buffer_of(bqb_ctx *c, int id, int node, char **ptr, size_t *bytes)
     case 27: *ptr = (char *)(p->nvec + node); *bytes = sizeof(int); break;
     case 28: *ptr = (char *)(p->dgamma + (size_t)node * d->max_m); *bytes = sizeof(double) * d->max_m; break;
     case 29: *ptr = (char *)(p->sa_val + (size_t)node * BQB_SA_CHAINS); *bytes = sizeof(double) * BQB_SA_CHAINS; break;
+    case 30: *ptr = (char *)(p->ycen + (size_t)node * d->ld); *bytes = sizeof(double) * d->ld; break;
     default: bqb_set_error("unknown buffer %d", id); return -1;
     }
     return 0;

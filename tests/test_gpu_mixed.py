@@ -122,6 +122,83 @@ def test_certificate_is_exactly_valid_small(slices):
         assert _psd_exact(Z), b
 
 
+def _bt_pattern(n, cut):
+    """B'(e_c) of one triangle cut, exactly: the oracle's op_Bt of a zero
+    matrix with multiplier 1 gives -B'(1), entries 0 or +-1/2."""
+    X = np.zeros((n, n), order="F")
+    X = O.op_Bt(X, O.cut_arrays([cut]), np.ones(1))
+    return [[-Fraction(X[i, j]) for j in range(n)] for i in range(n)]
+
+
+@pytest.mark.parametrize("slices", [0, 4])
+def test_round_end_certificate_is_exactly_valid(slices):
+    """The certificate computed at the end of each outer round for the
+    bundle center (y of the center's evaluation, its multipliers and cuts)
+    is exactly valid: Diag(y) - L + B'gamma + ((f_cert - sum(gamma) - e'y)/n) I
+    is psd in rational arithmetic.  The node runs phase by phase so the
+    check sees the cut set before the round's separation changes it."""
+    n0 = 14
+    SP = O.build_L(I.gnp_adj(n0, 0.6, 5, weights="int10"))
+    P = api.make_params(include_Pent=0, include_Hepta=0)
+    P.TriIneq = 140
+    ctx = api.BoundContext(SP, params=P, max_nodes=1)
+    ctx.set_precision(slices=slices, certify=1)
+    z = np.zeros(n0 - 1, np.int32)
+    ctx.push(z, z, LB=0.0, diff=1e9, is_root=0, seed=3)
+    ctx.run(api.PH["BASIC_TRAIN"])
+    PL, fv = O.create_subproblem(SP, z, z)
+    n = PL.shape[0]
+    checked = 0
+    for rnd in range(3):
+        st = ctx.state(0)
+        if not st.active:
+            break
+        ctx.run(api.PH["OUTER_BEGIN"])
+        for b in range(ctx.state(0).bdl_left):
+            ctx.run(api.PH["BUNDLE_NEXT"], b)
+            ctx.run(api.PH["QP"], 0)
+            ctx.run(api.PH["OPBT"], 1)
+            ctx.run(api.PH["IPM_L0"])
+            ctx.run(api.PH["STEP"])
+        stale = ctx.state(0).cert_stale
+        ctx.run(api.PH["CERT_MASK"])
+        ctx.run(111)
+        st = ctx.state(0)
+        assert st.cert_stale == 0 and st.status == 0
+        if stale:
+            m = st.NIneq
+            y = [Fraction(v) for v in ctx.get("ycen", 0, np.float64, n)]
+            # the certificate is that of max(gamma, 0) (rounding-level
+            # negatives of the bundle's multipliers are dropped)
+            gam = np.maximum(ctx.get("gamma", 0, np.float64, m), 0.0)
+            tri = ctx.get("tri", 0, np.int32, 4 * m).reshape(m, 4)
+            Z = [[(y[i] if i == j else 0) - Fraction(PL[i, j]) for j in range(n)] for i in range(n)]
+            for c in range(m):
+                if gam[c] == 0:
+                    continue
+                pat = _bt_pattern(n, tuple(int(v) for v in tri[c]))
+                g = Fraction(gam[c])
+                for i in range(n):
+                    for j in range(n):
+                        if pat[i][j]:
+                            Z[i][j] += g * pat[i][j]
+            sh = (Fraction(st.f_cert) - sum(Fraction(v) for v in gam) - sum(y)) / n
+            for i in range(n):
+                Z[i][i] += sh
+            assert _psd_exact(Z), (rnd, st.f_cert)
+            assert st.f_cert >= st.f                       # certified value at least f
+            checked += 1
+        # rest of the round: heuristic, tests, separation, rebuild
+        for ph in ("OUTER_A", "HEUR", "OUTER_B", "TRISEP", "SEP_PENT", "PENT", "SEP_HEPT", "HEPT", "OUTER_C",
+                   "CSR"):
+            ctx.run(api.PH[ph])
+        ctx.run(api.PH["OPBT"], 0)
+        ctx.run(api.PH["IPM_L0"])
+        ctx.run(api.PH["REBUILD"])
+    ctx.close()
+    assert checked >= 1
+
+
 def _bound_both(SP, P, xf, sx, seeds, slices):
     n0 = SP.shape[0]
     ctx = api.BoundContext(SP, params=P, max_nodes=xf.shape[0])

@@ -109,8 +109,12 @@ PROOF_FORMS = (
     ("plain", dict(batch=1)),
     ("batched", dict(batch=64)),
     ("batched_inherited", dict(batch=64, warm=True)),
-    ("batched_inherited_mixed", dict(batch=64, warm=True, slices=4)),
-    ("batched_inherited_mixed_learned", dict(batch=64, warm=True, slices=4, learned=True)),
+    # mixed: the automatic precision (per size class the fastest measured
+    # solve -- FP64 products at n = 60 -- with certified bounds)
+    ("batched_inherited_mixed", dict(batch=64, warm=True, slices=-1)),
+    ("batched_inherited_mixed_learned", dict(batch=64, warm=True, slices=-1, learned=True)),
+    # the IPM products forced onto the bf16 matrix cores (4 slices)
+    ("batched_inherited_bf16", dict(batch=64, warm=True, slices=4)),
 )
 
 
@@ -298,6 +302,35 @@ def mixed_section(api, I, seed, nodes=64):
     out["roofline_fp64"] = {"kernel": "ipm_blk_kernel (FP64)", "bound": "mfma", "achieved": round(ach, 4),
                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 5),
                             "launches": launches}
+    # where the automatic table takes bf16 products: the 193-224 class
+    # (G(224, 0.1), 64 nodes with 0-14 fixings, sizes 210-224), automatic
+    # precision against the FP64 default table
+    SP2 = api.maxcut_L(I.gnp_adj(224, 0.1, seed, weights="int10"))
+    P2 = api.make_params()
+    P2.TriIneq = 2240
+    xf2, sx2 = I.random_fixings(224, 14, nodes, seed + 2)
+    cls = {"workload": f"G(224, 0.1) weights +-1..+-10, {nodes} nodes with 0-14 fixings (sizes 210-224) as "
+                       "one batch after the root bound; reference params",
+           "auto_table_224": list(api.ipm_auto_table()[224])}
+    bounds = {}
+    for mode, sl in (("fp64", 0), ("auto", -1)):
+        ctx = api.BoundContext(SP2, params=P2, max_nodes=nodes)
+        ctx.set_precision(slices=sl, certify=0 if sl == 0 else 1)
+        root = ctx.bound(np.zeros((1, 223), np.int32), np.zeros((1, 223), np.int32), LB=0.0, is_root=1,
+                         seeds=np.uint64(seed))
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = ctx.bound(xf2, sx2, LB=float(root["LB"][0]), diff=float(root["diff"][0]), is_root=0, seeds=seeds)
+        dt = time.perf_counter() - t0
+        ctx.close()
+        if np.any(r["status"]):
+            raise SystemExit(f"class 224 {mode}: node failures {np.unique(r['status'])}")
+        bounds[mode] = r["bound"]
+        cls[f"{mode}_bounds_per_s"] = round(nodes / dt, 3)
+    cls["speedup_auto_vs_fp64"] = round(cls["auto_bounds_per_s"] / cls["fp64_bounds_per_s"], 3)
+    rel2 = np.abs(bounds["auto"] - bounds["fp64"]) / np.abs(bounds["fp64"])
+    cls["rel_bound_diff_auto_vs_fp64"] = {"median": float(np.median(rel2)), "max": float(rel2.max())}
+    out["class_224"] = cls
     return out
 
 

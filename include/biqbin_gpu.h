@@ -281,6 +281,20 @@ int bqb_set_ipm_tile_table(const int *tiles);
  * the default).  ms_out (optional, 2 per class) receives the medians
  * (register, blocked); tiles (optional) the resulting table. */
 int bqb_autotune_ipm(int nmin, int nmax, int nodes, int reps, double *ms_out, int *tiles);
+/* Tile and product precision per class for contexts whose precision is
+ * automatic (bqb_ctx_set_precision with slices = -1): tiles as above,
+ * slices 0 (FP64 products) or 2..4 (bf16 matrix-core products).  Defaults:
+ * FP64 products in every class but 193-224, where the register tile with
+ * 4 bf16 slices is the fastest measured solve. */
+int bqb_ipm_auto_table(int *tiles, int *slices);
+int bqb_set_ipm_auto_table(const int *tiles, const int *slices);
+int bqb_ipm_auto_for(int nmax, int *slices);
+/* The autotuner for automatic precision: per class in [nmin, nmax], times
+ * the register and blocked solves with FP64 products and with 4 bf16
+ * slices (ms_out, optional: 4 per class in that order; a variant a class
+ * cannot run reads 0) and installs the fastest in the automatic table (a
+ * variant replaces the current entry only if at least 5 % faster). */
+int bqb_autotune_ipm_auto(int nmin, int nmax, int nodes, int reps, double *ms_out, int *tiles, int *slices);
 
 /* ---- batch: the full node bound (SDPbound) ----------------------------- */
 /* A context holds the device arenas of up to max_nodes B&B nodes of one
@@ -406,7 +420,8 @@ int bqb_ctx_set_heuristic(bqb_ctx *c, bqb_heuristic_fn fn, void *user);
 /* Precision of a context's bounds (no counterpart in the reference; call
  * while no node is in flight): slices = 0 runs the IPM products in FP64 (the
  * reference's arithmetic), 2..4 on the bf16 matrix cores as in
- * bqb_ipm_batch_mixed.  certify = 1 (implied by slices > 0) makes every
+ * bqb_ipm_batch_mixed, -1 takes the tile and products of the context's size
+ * class from the automatic table (bqb_ipm_auto_table) and certifies.  certify = 1 (implied by slices > 0) makes every
  * value the node prunes on and returns a certified upper bound: the basic
  * bound, and at the end of every outer round the bundle center if it moved
  * (serious steps, bundle.c:93-135) since its last certificate, get a

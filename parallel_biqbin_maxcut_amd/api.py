@@ -109,6 +109,35 @@ def autotune_ipm(nmin=1, nmax=256, nodes=64, reps=3):
     return dict(zip(TILE_CLASS_LD, (int(v) for v in t))), timed
 
 
+def ipm_auto_table():
+    """{slab ld of the size class: (IPM tile, slices)} of contexts with
+    automatic precision (slices 0: FP64 products)."""
+    t = np.zeros(len(TILE_CLASS_LD), dtype=np.int32)
+    sl = np.zeros(len(TILE_CLASS_LD), dtype=np.int32)
+    N.lib().bqb_ipm_auto_table(_i(t), _i(sl))
+    return {ld: (int(t[c]), int(sl[c])) for c, ld in enumerate(TILE_CLASS_LD)}
+
+
+def set_ipm_auto_table(table):
+    t = np.array([int(table[ld][0]) for ld in TILE_CLASS_LD], dtype=np.int32)
+    sl = np.array([int(table[ld][1]) for ld in TILE_CLASS_LD], dtype=np.int32)
+    N.check(N.lib().bqb_set_ipm_auto_table(_i(t), _i(sl)))
+
+
+def autotune_ipm_auto(nmin=1, nmax=256, nodes=64, reps=3):
+    """Time register / blocked solves with FP64 and bf16 products for every
+    size class in [nmin, nmax] and install the fastest in the automatic
+    table (bqb_autotune_ipm_auto).  Returns (table, {ld: (register FP64,
+    blocked FP64, register bf16, blocked bf16) ms})."""
+    ms = np.zeros(4 * len(TILE_CLASS_LD))
+    t = np.zeros(len(TILE_CLASS_LD), dtype=np.int32)
+    sl = np.zeros(len(TILE_CLASS_LD), dtype=np.int32)
+    N.check(N.lib().bqb_autotune_ipm_auto(nmin, nmax, nodes, reps, _d(ms), _i(t), _i(sl)))
+    timed = {ld: tuple(round(float(ms[4 * c + v]), 4) for v in range(4))
+             for c, ld in enumerate(TILE_CLASS_LD) if ms[4 * c] > 0}
+    return {ld: (int(t[c]), int(sl[c])) for c, ld in enumerate(TILE_CLASS_LD)}, timed
+
+
 def mx_product(A, B, slices=3):
     """C = A B through the sliced bf16 matrix-core product (diagnostic)."""
     A = np.asfortranarray(A, dtype=np.float64)

@@ -97,6 +97,49 @@ extern "C" int bqb_set_ipm_tile_table(const int *tiles)
 
 extern "C" int bqb_ipm_class_ld(int c) { return c >= 0 && c < BQB_TILE_CLASSES ? k_class_ld[c] : 0; }
 
+// Tile and product precision per class for contexts whose precision is
+// automatic (bqb_ctx_set_precision with slices = -1): the fastest of the
+// register / blocked kernels with FP64 or bf16 products.  Defaults from a
+// measurement of 128-256 basic solves per class on MI355X: bf16 products
+// lose below 193 (the products are a small share of a solve there, and
+// slicing costs more than it saves) and at 225-256 (the blocked FP64 solve
+// is fastest); at 193-224 the register tile with bf16 products wins (27.0
+// vs 34.7 ms, FP64 register tile, and 36.5 ms, blocked).
+// bqb_autotune_ipm_auto measures and replaces them.
+static int g_auto_tile[BQB_TILE_CLASSES] = {1604, 1605, 1606, 1607, 3204, 3205, 3206, 3207, 25601};
+static int g_auto_slices[BQB_TILE_CLASSES] = {0, 0, 0, 0, 0, 0, 0, 4, 0};
+
+extern "C" int bqb_ipm_auto_table(int *tiles, int *slices)
+{
+    for (int c = 0; c < BQB_TILE_CLASSES; ++c) {
+        if (tiles) tiles[c] = g_auto_tile[c];
+        if (slices) slices[c] = g_auto_slices[c];
+    }
+    return BQB_TILE_CLASSES;
+}
+
+extern "C" int bqb_set_ipm_auto_table(const int *tiles, const int *slices)
+{
+    for (int c = 0; c < BQB_TILE_CLASSES; ++c) {
+        const int ld = k_class_ld[c], t = tiles[c], s = slices[c];
+        if (t != bqb_ipm_reg_tile_for_ld(ld) && t != ld * 100 + 1) return -1;
+        if (s != 0 && (s < 2 || s > BQB_MIX_MAX_SLICES)) return -1;
+    }
+    for (int c = 0; c < BQB_TILE_CLASSES; ++c) {
+        g_auto_tile[c] = tiles[c];
+        g_auto_slices[c] = slices[c];
+    }
+    return 0;
+}
+
+extern "C" int bqb_ipm_auto_for(int nmax, int *slices)
+{
+    const int c = tile_class(nmax);
+    if (c < 0 || nmax < 1) return 0;
+    if (slices) *slices = g_auto_slices[c];
+    return g_auto_tile[c];
+}
+
 extern "C" int bqb_ipm_tile_reg_for(int nmax)
 {
     if (nmax <= 64) return 1604;

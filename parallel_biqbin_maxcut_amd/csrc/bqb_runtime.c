@@ -396,16 +396,17 @@ static int cmp_double(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
-/* Median launch time of the batch with the given tile forced. */
-static int time_tile(int tile, int nodes, const int *nv, const double *L, double *X, double *phi, int *it, int *tr,
-                     int *st, int reps, double *med)
+/* Median launch time of the batch with the given tile forced and the
+ * products in FP64 (slices 0) or on the bf16 matrix cores. */
+static int time_tile(int tile, int slices, int nodes, const int *nv, const double *L, double *X, double *phi, int *it,
+                     int *tr, int *st, int reps, double *med)
 {
     double ms[16];
     if (reps > 16) reps = 16;
     bqb_set_ipm_tile(tile);
     for (int r = -1; r < reps; ++r) {
         double t = 0.0;
-        if (ipm_batch_impl(nodes, nv, L, X, phi, it, tr, st, NULL, NULL, 0, 0, NULL, 0, NULL, &t)) {
+        if (ipm_batch_impl(nodes, nv, L, X, phi, it, tr, st, NULL, NULL, 0, 0, NULL, slices, NULL, &t)) {
             bqb_set_ipm_tile(0);
             return -1;
         }
@@ -415,6 +416,31 @@ static int time_tile(int tile, int nodes, const int *nv, const double *L, double
     qsort(ms, (size_t)reps, sizeof(double), cmp_double);
     *med = ms[reps / 2];
     return 0;
+}
+
+/* The subproblems an autotuner times for class ld: `nodes` Max-Cut L of
+ * G(ld, 0.3) with weights +-1 (the class's largest size). */
+static int tune_batch(int ld, int nodes, double **L, int **nv)
+{
+    const int n = ld;
+    double *adj = calloc((size_t)n * n, sizeof(double));
+    *L = malloc(sizeof(double) * (size_t)n * n * nodes);
+    *nv = malloc(sizeof(int) * nodes);
+    if (!adj || !*L || !*nv) { free(adj); bqb_set_error("malloc"); return -1; }
+    uint64_t z = 0x9E3779B97F4A7C15ull * (uint64_t)(ld + 1);
+    int rc = 0;
+    for (int b = 0; b < nodes && !rc; ++b) {
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                z = z * 6364136223846793005ull + 1442695040888963407ull;
+                const double w = ((z >> 33) % 10 < 3) ? (((z >> 20) & 1) ? 1.0 : -1.0) : 0.0;
+                adj[i + (size_t)j * n] = adj[j + (size_t)i * n] = w;
+            }
+        if (bqb_build_L(n, adj, *L + (size_t)b * n * n)) rc = -1;
+        (*nv)[b] = n;
+    }
+    free(adj);
+    return rc;
 }
 
 int bqb_autotune_ipm(int nmin, int nmax, int nodes, int reps, double *ms_out, int *tiles)
@@ -427,38 +453,63 @@ int bqb_autotune_ipm(int nmin, int nmax, int nodes, int reps, double *ms_out, in
     for (int c = 0; c < BQB_TILE_CLASSES && !rc; ++c) {
         const int ld = bqb_ipm_class_ld(c), lo = c ? bqb_ipm_class_ld(c - 1) + 1 : 1;
         if (ld < nmin || lo > nmax) continue;
-        /* the class's largest subproblems: G(ld, 0.3), weights +-1, as Max-Cut L */
-        const int n = ld;
-        double *adj = calloc((size_t)n * n, sizeof(double));
-        double *L = malloc(sizeof(double) * (size_t)n * n * nodes), *X = malloc(sizeof(double) * (size_t)n * n * nodes);
-        double *phi = malloc(sizeof(double) * nodes);
-        int *nv = malloc(sizeof(int) * nodes), *it = malloc(sizeof(int) * nodes), *tr = malloc(sizeof(int) * nodes),
+        double *L = NULL, *X = malloc(sizeof(double) * (size_t)ld * ld * nodes), *phi = malloc(sizeof(double) * nodes);
+        int *nv = NULL, *it = malloc(sizeof(int) * nodes), *tr = malloc(sizeof(int) * nodes),
             *st = malloc(sizeof(int) * nodes);
-        if (!adj || !L || !X || !phi || !nv || !it || !tr || !st) { bqb_set_error("malloc"); rc = -1; }
-        uint64_t z = 0x9E3779B97F4A7C15ull * (uint64_t)(ld + 1);
-        for (int b = 0; b < nodes && !rc; ++b) {
-            for (int i = 0; i < n; ++i)
-                for (int j = i + 1; j < n; ++j) {
-                    z = z * 6364136223846793005ull + 1442695040888963407ull;
-                    const double w = ((z >> 33) % 10 < 3) ? (((z >> 20) & 1) ? 1.0 : -1.0) : 0.0;
-                    adj[i + (size_t)j * n] = adj[j + (size_t)i * n] = w;
-                }
-            if (bqb_build_L(n, adj, L + (size_t)b * n * n)) rc = -1;
-            nv[b] = n;
-        }
+        if (!X || !phi || !it || !tr || !st) { bqb_set_error("malloc"); rc = -1; }
+        if (!rc) rc = tune_batch(ld, nodes, &L, &nv);
         double mr = 0.0, mb = 0.0;
         const int reg = bqb_ipm_reg_tile_for_ld(ld), blk = ld * 100 + 1;
-        if (!rc) rc = time_tile(reg, nodes, nv, L, X, phi, it, tr, st, reps, &mr);
-        if (!rc) rc = time_tile(blk, nodes, nv, L, X, phi, it, tr, st, reps, &mb);
+        if (!rc) rc = time_tile(reg, 0, nodes, nv, L, X, phi, it, tr, st, reps, &mr);
+        if (!rc) rc = time_tile(blk, 0, nodes, nv, L, X, phi, it, tr, st, reps, &mb);
         if (!rc) {
             const int cur = table[c], other = cur == reg ? blk : reg;
             const double tcur = cur == reg ? mr : mb, toth = cur == reg ? mb : mr;
             if (toth < 0.95 * tcur) table[c] = other;
             if (ms_out) { ms_out[2 * c] = mr; ms_out[2 * c + 1] = mb; }
         }
-        free(adj); free(L); free(X); free(phi); free(nv); free(it); free(tr); free(st);
+        free(L); free(X); free(phi); free(nv); free(it); free(tr); free(st);
     }
     if (!rc && bqb_set_ipm_tile_table(table)) { bqb_set_error("autotune: table rejected"); rc = -1; }
     if (tiles) bqb_ipm_tile_table(tiles);
+    return rc;
+}
+
+int bqb_autotune_ipm_auto(int nmin, int nmax, int nodes, int reps, double *ms_out, int *tiles, int *slices)
+{
+    if (nodes < 1 || reps < 1 || nmin < 1 || nmax < nmin) { bqb_set_error("autotune: bad arguments"); return -1; }
+    if (bqb_ensure_init()) return -1;
+    int ttab[BQB_TILE_CLASSES], stab[BQB_TILE_CLASSES];
+    bqb_ipm_auto_table(ttab, stab);
+    int rc = 0;
+    for (int c = 0; c < BQB_TILE_CLASSES && !rc; ++c) {
+        const int ld = bqb_ipm_class_ld(c), lo = c ? bqb_ipm_class_ld(c - 1) + 1 : 1;
+        if (ld < nmin || lo > nmax) continue;
+        double *L = NULL, *X = malloc(sizeof(double) * (size_t)ld * ld * nodes), *phi = malloc(sizeof(double) * nodes);
+        int *nv = NULL, *it = malloc(sizeof(int) * nodes), *tr = malloc(sizeof(int) * nodes),
+            *st = malloc(sizeof(int) * nodes);
+        if (!X || !phi || !it || !tr || !st) { bqb_set_error("malloc"); rc = -1; }
+        if (!rc) rc = tune_batch(ld, nodes, &L, &nv);
+        /* variants: register / blocked, FP64 / 4 bf16 slices */
+        const int vt[4] = {bqb_ipm_reg_tile_for_ld(ld), ld * 100 + 1, bqb_ipm_reg_tile_for_ld(ld), ld * 100 + 1};
+        const int vs[4] = {0, 0, 4, 4};
+        double ms[4] = {0, 0, 0, 0};
+        for (int v = 0; v < 4 && !rc; ++v) rc = time_tile(vt[v], vs[v], nodes, nv, L, X, phi, it, tr, st, reps, &ms[v]);
+        if (!rc) {
+            int cur = 0;
+            for (int v = 0; v < 4; ++v)
+                if (vt[v] == ttab[c] && vs[v] == stab[c]) cur = v;
+            int best = cur;
+            for (int v = 0; v < 4; ++v)
+                if (ms[v] < 0.95 * ms[cur] && ms[v] < ms[best]) best = v;
+            ttab[c] = vt[best];
+            stab[c] = vs[best];
+            if (ms_out)
+                for (int v = 0; v < 4; ++v) ms_out[4 * c + v] = ms[v];
+        }
+        free(L); free(X); free(phi); free(nv); free(it); free(tr); free(st);
+    }
+    if (!rc && bqb_set_ipm_auto_table(ttab, stab)) { bqb_set_error("autotune: table rejected"); rc = -1; }
+    bqb_ipm_auto_table(tiles, slices);
     return rc;
 }

@@ -59,6 +59,7 @@ struct bqb_ctx {
     BqbParams params;
     int max_nodes;
     int tile;           /* IPM register tile (P*100 + R) */
+    int tile0;          /* the tile chosen at creation (FP64 / explicit precision) */
     int heur_tile;      /* heuristic tile (P*100 + R) */
     int spl_int;        /* SP_L integer-valued (|entries| <= 2^20) */
     void *arena;
@@ -261,7 +262,7 @@ bqb_ctx *bqb_ctx_create(int n0, const double *SP_L, const BqbParams *params, int
         return NULL;
     }
     c->max_nodes = max_nodes;
-    c->tile = tile;
+    c->tile = c->tile0 = tile;
     c->heur_tile = heur_tile;
     c->spl_int = 1;
     for (size_t e = 0; e < (size_t)n0 * n0 && c->spl_int; ++e)
@@ -1615,11 +1616,24 @@ int bqb_ctx_learn_records(bqb_ctx *c, int *count, float *feat, int *key)
 
 int bqb_ctx_set_precision(bqb_ctx *c, int slices, int certify)
 {
-    if (slices < 0 || slices == 1 || slices > BQB_MIX_MAX_SLICES) {
-        bqb_set_error("slices = %d: 0 (FP64 products) or 2..%d", slices, BQB_MIX_MAX_SLICES);
+    if (slices < -1 || slices == 1 || slices > BQB_MIX_MAX_SLICES) {
+        bqb_set_error("slices = %d: 0 (FP64 products), 2..%d, or -1 (automatic)", slices, BQB_MIX_MAX_SLICES);
         return -1;
     }
     if (bqb_ctx_busy(c) > 0) { bqb_set_error("set_precision: nodes are in flight"); return -1; }
+    int tile = c->tile0;
+    if (slices < 0) {
+        /* the class's entry of the automatic table; its slabs are those of
+         * the context's class, so only the kernel changes */
+        int s = 0;
+        const int t = bqb_ipm_auto_for(c->d.n0, &s);
+        const int tld = (t % 100 == 1) ? t / 100 : (t / 100) * (t % 100);
+        if (!t || tld > c->d.ld) { bqb_set_error("set_precision: no automatic tile for n0 = %d", c->d.n0); return -1; }
+        tile = t;
+        slices = s;
+        certify = 1;
+    }
+    c->tile = tile;
     c->d.mix_slices = slices;
     c->d.certify = (certify || slices > 0) ? 1 : 0;
     c->d.work_slabs = BQB_IPM_WORK_SLABS(slices);

@@ -299,3 +299,52 @@ def test_three_slices_fall_back_to_fp64_where_needed():
         print(f"node {b}: status {r['status'][b]}, FP64 re-solve {retried}", flush=True)
         assert r["status"][b] == 0 and o.status == 0
         assert abs(r["bound"][b] - o.bound) <= 1e-4 * abs(o.bound)
+
+
+def test_automatic_precision_follows_the_class_table():
+    """slices = -1 takes the context's class entry of the automatic table:
+    at n0 = 60 FP64 products with certificates (bitwise the certified FP64
+    context), at n0 = 220 the 32 x 7 register tile with 4 bf16 slices
+    (bitwise the explicit form, and within 1e-4 of the FP64 oracle)."""
+    table = api.ipm_auto_table()
+    assert table[224] == (3207, 4)
+    assert all(table[ld][1] == 0 for ld in (64, 80, 96, 112, 128, 160, 192, 256))
+    P = api.make_params(include_Pent=0, include_Hepta=0, max_outer_iter=2, extra_iter=0)
+    for n0, nodes, seed in ((60, 4, 3), (220, 2, 4)):
+        SP = O.build_L(I.gnp_adj(n0, 0.1 if n0 > 100 else 0.5, seed, weights="int10"))
+        P.TriIneq = 10 * n0
+        xf, sx = I.random_fixings(n0, 5, nodes, seed + 1)
+        seeds = np.arange(1, nodes + 1, dtype=np.uint64)
+        ctx = api.BoundContext(SP, params=P, max_nodes=nodes)
+        ctx.set_precision(slices=-1)
+        auto = ctx.bound(xf, sx, LB=0.0, diff=0.0, is_root=0, seeds=seeds)
+        ctx.close()
+        tile, sl = table[224] if n0 > 192 else (api.N.lib().bqb_ipm_tile_for(n0), 0)
+        api.N.lib().bqb_set_ipm_tile(tile)
+        ctx = api.BoundContext(SP, params=P, max_nodes=nodes)
+        api.N.lib().bqb_set_ipm_tile(0)
+        ctx.set_precision(slices=sl, certify=1)
+        expl = ctx.bound(xf, sx, LB=0.0, diff=0.0, is_root=0, seeds=seeds)
+        ctx.close()
+        assert np.all(auto["status"] == 0) and np.array_equal(auto["bound"], expl["bound"]), (n0, auto["bound"],
+                                                                                            expl["bound"])
+        if n0 > 192:
+            d = {f: getattr(P, f) for f, _ in api.BiqBinParameters._fields_}
+            for b in range(nodes):
+                o = O.sdpbound(SP, xf[b], sx[b], 0.0, 0.0, 0, int(seeds[b]), O.default_params(**d))[0]
+                assert abs(auto["bound"][b] - o.bound) <= 1e-4 * abs(o.bound), (b, auto["bound"][b], o.bound)
+
+
+def test_autotuner_for_automatic_precision():
+    """The autotuner times the four variants of the classes it covers and
+    installs valid entries (restored afterwards)."""
+    default = api.ipm_auto_table()
+    try:
+        table, timed = api.autotune_ipm_auto(nmin=200, nmax=224, nodes=4, reps=1)
+        assert set(timed) == {224}
+        assert all(v > 0 for v in timed[224])
+        assert table[224][0] in (3207, 22401) and table[224][1] in (0, 4)
+        for ld in (64, 80, 96, 112, 128, 160, 192, 256):
+            assert table[ld] == default[ld]
+    finally:
+        api.set_ipm_auto_table(default)

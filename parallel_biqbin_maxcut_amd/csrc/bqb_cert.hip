@@ -101,10 +101,10 @@ __device__ __forceinline__ double red_max(double v, double *red)
 
 // The certificate of one node: returns RU(sum_gamma_ru + e'y + n (d + r)).
 // L (ld) the node's objective, y its dual vector, cs the cuts (NULL: none).
-// Zt, R: two ld x ld work slabs.  rs: LDS of 2 ld doubles.
+// Zt: an ld x ld work slab; R: the factor's storage with leading dimension
+// ldr (a second work slab, or LDS for small n).  rs: LDS of 2 ld doubles.
 __device__ double certify(int n, int ld, const double *L, const double *y, const CutSource *cs,
-                          double sum_gamma_ru, double *Zt, double *R, double *rs, double *red,
-                          int *s_fail, double *s_piv)
+                          double sum_gamma_ru, double *Zt, double *R, int ldr, double *rs, double *red)
 {
     const int tid = threadIdx.x;
     // 1. Z~ and the row sums of the enclosure widths
@@ -133,8 +133,11 @@ __device__ double certify(int n, int ld, const double *L, const double *y, const
     zmax = red_max(zmax, red);
 
     // 2. Cholesky of Z~ + delta I, delta = 0, then 1e-12, 1e-10, 1e-8 of max |Z~_ii|
+    // (right-looking; the trailing update of step k is spread over a 16 x 16
+    // thread grid, every entry updated in k order as by a row loop)
     double dshift = 0.0, delta = 0.0;
     bool ok = false;
+    const int gi = tid % 16, gj = tid / 16;
     for (int att = 0; att < 4 && !ok; ++att) {
         delta = (att == 0) ? 0.0 : zmax * ((att == 1) ? 1e-12 : (att == 2) ? 1e-10 : 1e-8);
         double dsh = 0.0;
@@ -146,31 +149,26 @@ __device__ double certify(int n, int ld, const double *L, const double *y, const
                     dsh = fmax(dsh, bqb_sub_ru(vd, v));
                     v = vd;
                 }
-                R[i + (size_t)j * ld] = v;
+                R[i + (size_t)j * ldr] = v;
             }
-        dshift = red_max(dsh, red);
-        if (tid == 0) *s_fail = 0;
-        __syncthreads();
+        dshift = red_max(dsh, red);      // (its barriers also publish R)
+        ok = true;
         for (int k = 0; k < n; ++k) {
-            if (tid == 0) {
-                const double dk = R[k + (size_t)k * ld];
-                if (!(dk > 0.0)) *s_fail = 1;
-                *s_piv = sqrt(dk);
-                R[k + (size_t)k * ld] = *s_piv;
-            }
+            // every thread reads the pivot (the same value), so the test
+            // and the square root need no broadcast
+            const double dk = R[k + (size_t)k * ldr];
+            if (!(dk > 0.0)) { ok = false; break; }
+            const double rk = sqrt(dk);
+            for (int i = k + 1 + tid; i < n; i += CE) R[i + (size_t)k * ldr] = R[i + (size_t)k * ldr] / rk;
             __syncthreads();
-            if (*s_fail) break;
-            const double rk = *s_piv;
-            for (int i = k + 1 + tid; i < n; i += CE) R[i + (size_t)k * ld] = R[i + (size_t)k * ld] / rk;
-            __syncthreads();
-            for (int i = k + 1 + tid; i < n; i += CE) {
-                const double rik = R[i + (size_t)k * ld];
-                for (int j = k + 1; j <= i; ++j)
-                    R[i + (size_t)j * ld] = fma(-rik, R[j + (size_t)k * ld], R[i + (size_t)j * ld]);
+            if (tid == 0) R[k + (size_t)k * ldr] = rk;     // read again only after the loop
+            for (int i = k + 1 + gi; i < n; i += 16) {
+                const double rik = R[i + (size_t)k * ldr];
+                for (int j = k + 1 + gj; j <= i; j += 16)
+                    R[i + (size_t)j * ldr] = fma(-rik, R[j + (size_t)k * ldr], R[i + (size_t)j * ldr]);
             }
             __syncthreads();
         }
-        ok = !*s_fail;
         __syncthreads();
     }
 
@@ -192,7 +190,7 @@ __device__ double certify(int n, int ld, const double *L, const double *y, const
             const int b = e - a * (a + 1) / 2;
             double hi = 0.0, lo = 0.0;
             for (int k = 0; k <= b; ++k) {
-                const double ra = R[a + (size_t)k * ld], rb = R[b + (size_t)k * ld];
+                const double ra = R[a + (size_t)k * ldr], rb = R[b + (size_t)k * ldr];
                 hi = bqb_add_ru(hi, bqb_mul_ru(ra, rb));
                 lo = bqb_add_rd(lo, bqb_mul_rd(ra, rb));
             }
@@ -200,14 +198,14 @@ __device__ double certify(int n, int ld, const double *L, const double *y, const
             if (a == b) tgt = tgt + delta;                  // the entry that was factored
             const double e2 = fmax(bqb_sub_ru(hi, tgt), bqb_sub_ru(tgt, lo));
             if (a == b) rs[ld + a] = e2;
-            else R[b + (size_t)a * ld] = e2;
+            else R[b + (size_t)a * ldr] = e2;
         }
         __syncthreads();
         double rmax = 0.0;
         for (int i = tid; i < n; i += CE) {
             double acc = bqb_add_ru(rs[i], rs[ld + i]);
-            for (int j = 0; j < i; ++j) acc = bqb_add_ru(acc, R[j + (size_t)i * ld]);
-            for (int j = i + 1; j < n; ++j) acc = bqb_add_ru(acc, R[i + (size_t)j * ld]);
+            for (int j = 0; j < i; ++j) acc = bqb_add_ru(acc, R[j + (size_t)i * ldr]);
+            for (int j = i + 1; j < n; ++j) acc = bqb_add_ru(acc, R[i + (size_t)j * ldr]);
             rmax = fmax(rmax, acc);
         }
         rmax = red_max(rmax, red);
@@ -227,20 +225,28 @@ __device__ double certify(int n, int ld, const double *L, const double *y, const
     return bqb_add_ru(bqb_add_ru(sum_gamma_ru, sy), bqb_mul_ru((double)n, lam));
 }
 
+// The factor of the certificate lives in LDS when the slabs are at most NL
+// wide (NL = 64 or 128: 32 or 128 KB), else in the node's second work slab.
+template <int NL>
+struct CertR {
+    __device__ __forceinline__ static double *ptr(double *slab, double *lds) { return NL ? lds : slab; }
+};
+
 // ---- the IPM entry point: certificates of e'y for Z = Diag(y) - L ----
+template <int NL>
 __global__ __launch_bounds__(CE) void cert_ipm_kernel(const int *nvec, const double *L, const double *y,
                                                       long ystride, double *work, int ld, int wslabs,
                                                       const int *status, double *out)
 {
-    __shared__ double rs[512], red[CE / 64], s_piv;
-    __shared__ int s_fail;
+    __shared__ double rs[512], red[CE / 64];
+    __shared__ double s_R[NL ? NL * NL : 1];
     const int node = blockIdx.x;
     const int n = nvec[node];
     if (status[node] || n < 1 || n > ld || n > 256) { if (threadIdx.x == 0) out[node] = INFINITY; return; }
     const size_t slab = (size_t)ld * ld;
     double *Zt = work + (size_t)node * wslabs * slab;
-    const double v = certify(n, ld, L + node * slab, y + node * ystride, nullptr, 0.0, Zt, Zt + slab, rs, red,
-                             &s_fail, &s_piv);
+    const double v = certify(n, ld, L + node * slab, y + node * ystride, nullptr, 0.0, Zt,
+                             CertR<NL>::ptr(Zt + slab, s_R), NL ? n : ld, rs, red);
     if (threadIdx.x == 0) out[node] = v;
 }
 
@@ -255,10 +261,11 @@ __global__ __launch_bounds__(CE) void cert_ipm_kernel(const int *nvec, const dou
 // mode 2: after the rebuild of a round that made the warm-started center
 // gamma the new f (y of the IPM on L - B'gamma).
 // The certified value replaces NodeState.f_cert.
+template <int NL>
 __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p, int mode)
 {
-    __shared__ double rs[512], red[CE / 64], s_piv;
-    __shared__ int s_fail;
+    __shared__ double rs[512], red[CE / 64];
+    __shared__ double s_R[NL ? NL * NL : 1];
     const int node = blockIdx.x;
     if (node >= d.nodes || !p.mask[node]) return;
     NodeState *sp = p.st + node;
@@ -287,7 +294,8 @@ __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p,
         for (int c = threadIdx.x; c < m; c += CE) part = bqb_add_ru(part, fmax(gam[c], 0.0));
         sg = red_ru(part, red);
     }
-    const double v = certify(n, d.ld, p.L + node * slab, y, csp, sg, Zt, Zt + slab, rs, red, &s_fail, &s_piv);
+    const double v = certify(n, d.ld, p.L + node * slab, y, csp, sg, Zt, CertR<NL>::ptr(Zt + slab, s_R),
+                             NL ? n : d.ld, rs, red);
     if (threadIdx.x == 0) { sp->f_cert = v; sp->cert_stale = 0; }
 }
 
@@ -299,13 +307,25 @@ extern "C" hipError_t bqb_launch_cert_ipm(const int *nvec, const double *L, cons
                                           double *work, int ld, int wslabs, const int *status, double *out,
                                           int nodes, hipStream_t s)
 {
-    hipLaunchKernelGGL(cert_ipm_kernel, dim3(nodes), dim3(CE), 0, s, nvec, L, y, ystride, work, ld, wslabs,
-                       status, out);
+    if (ld <= 64)
+        hipLaunchKernelGGL(cert_ipm_kernel<64>, dim3(nodes), dim3(CE), 0, s, nvec, L, y, ystride, work, ld, wslabs,
+                           status, out);
+    else if (ld <= 128)
+        hipLaunchKernelGGL(cert_ipm_kernel<128>, dim3(nodes), dim3(CE), 0, s, nvec, L, y, ystride, work, ld, wslabs,
+                           status, out);
+    else
+        hipLaunchKernelGGL(cert_ipm_kernel<0>, dim3(nodes), dim3(CE), 0, s, nvec, L, y, ystride, work, ld, wslabs,
+                           status, out);
     return hipGetLastError();
 }
 
 extern "C" hipError_t bqb_launch_cert_node(const BatchDims *d, const BatchPtrs *p, int mode, hipStream_t s)
 {
-    hipLaunchKernelGGL(cert_node_kernel, dim3(d->nodes), dim3(CE), 0, s, *d, *p, mode);
+    if (d->ld <= 64)
+        hipLaunchKernelGGL(cert_node_kernel<64>, dim3(d->nodes), dim3(CE), 0, s, *d, *p, mode);
+    else if (d->ld <= 128)
+        hipLaunchKernelGGL(cert_node_kernel<128>, dim3(d->nodes), dim3(CE), 0, s, *d, *p, mode);
+    else
+        hipLaunchKernelGGL(cert_node_kernel<0>, dim3(d->nodes), dim3(CE), 0, s, *d, *p, mode);
     return hipGetLastError();
 }

@@ -149,7 +149,7 @@ __device__ __forceinline__ int solve_lambda_wave(int k, const double *Q, const d
     return 0;
 }
 
-// The same algorithm for k <= K (K = 8, 12, 16, 24, 32) with the factor in
+// The same algorithm for k <= K (K = 8, 12, 16, 20, 24, 28, 32) with the factor in
 // registers: lane i holds row i of A = Q + Diag(z./lambda) and of its
 // Cholesky factor.  Pivots are read with v_readlane, each step's column is
 // stored to LDS (the back substitution reads the transpose from there).
@@ -402,7 +402,9 @@ __global__ __launch_bounds__(BT, (KM == KMAX) ? 2 : 3) void bundle_qp_kernel(Bat
                 st = (k <= 8)    ? solve_lambda_reg<8>(k, s_Q, s_c, s_lam, s_A, lane)
                    : (k <= 12) ? solve_lambda_reg<12>(k, s_Q, s_c, s_lam, s_A, lane)
                    : (k <= 16) ? solve_lambda_reg<16>(k, s_Q, s_c, s_lam, s_A, lane)
+                   : (k <= 20) ? solve_lambda_reg<20>(k, s_Q, s_c, s_lam, s_A, lane)
                    : (k <= 24) ? solve_lambda_reg<24>(k, s_Q, s_c, s_lam, s_A, lane)
+                   : (k <= 28) ? solve_lambda_reg<28>(k, s_Q, s_c, s_lam, s_A, lane)
                    : (k <= KR) ? solve_lambda_reg<KR>(k, s_Q, s_c, s_lam, s_A, lane)
                                : solve_lambda_wave(k, s_Q, s_c, s_lam, s_A, s_u, s_w, lane);
             if (lane == 0 && st) s_stat = st;

@@ -98,3 +98,25 @@ def test_round_one_runs_init_bundle_iter_uncapped():
     assert r["status"][0] == 0 and o.status == 0
     assert (s[0], s[1], s[2], s[3]) == (o.prune, o.giveup, o.outer_iters, o.ipm_calls)
     assert abs(r["bound"][0] - o.bound) <= 1e-6 * abs(o.bound), (r["bound"][0], o.bound)
+
+
+def test_reference_cut_capacities():
+    """Contexts holding the reference's cut capacities (MaxTriIneqAdded
+    50 000, pentagonal and heptagonal 50 000, src/biqbin.h:37-39) bound nodes
+    with results bitwise equal to the default capacities' when nothing
+    overflows (capacities change strides, not arithmetic)."""
+    n0 = 60
+    SP = O.build_L(I.gnp_adj(n0, 0.5, 8, weights="pm1"))
+    P = _params(n0, max_outer_iter=4, extra_iter=0)
+    xf, sx = I.random_fixings(n0, 8, 3, 9)
+    seeds = np.arange(1, 4, dtype=np.uint64)
+    out = []
+    for caps in ((0, 0, 0), (50000, 50000, 50000)):
+        ctx = api.BoundContext(SP, params=P, max_nodes=3, max_tri=caps[0], max_pent=caps[1], max_hept=caps[2])
+        r = ctx.bound(xf, sx, LB=0.0, diff=1e9, is_root=0, seeds=seeds)
+        ctx.close()
+        assert np.all(r["status"] == 0) and not np.any(r["stats"][:, 8] & (api.FLAG_TRI_TRUNC | api.FLAG_PENT_TRUNC
+                                                                        | api.FLAG_HEPT_TRUNC))
+        out.append(r)
+    assert np.array_equal(out[0]["bound"], out[1]["bound"])
+    assert np.array_equal(out[0]["stats"], out[1]["stats"])

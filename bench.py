@@ -190,6 +190,32 @@ def time_to_proof(api, I, seeds, cpu_seeds):
     out["fastest_form"] = best_form
     # bounds/s on the node stream of a real branch and bound (64 slots)
     out["bab_node_bounds_per_s_batched"] = round(nodes["batched"] / sum(secs["batched"]), 2)
+    # The root bound alone (one slot, the same parameters): every form
+    # starts with it and no child can start before it ends, so the plain
+    # total over the roots' total bounds any form's speedup, even one whose
+    # children cost nothing.
+    roots = []
+    for L in Ls:
+        B = L.shape[0] - 1
+        Pr = api.make_params(include_Pent=0, include_Hepta=0)
+        Pr.TriIneq = 10 * L.shape[0]               # src/process_input.c:135-136
+        z = np.zeros(B, np.int32)
+        for _ in range(2):                          # the first is untimed
+            ctx = api.BoundContext(L, params=Pr, max_nodes=1)
+            t0 = time.perf_counter()
+            ctx.push(z, z, LB=0.0, diff=0.0, is_root=1, seed=2020, tag=0)
+            while ctx.busy():
+                ctx.advance()
+            r = ctx.pop()
+            dt = time.perf_counter() - t0
+            ctx.close()
+        if r["status"]:
+            raise SystemExit(f"time-to-proof root bound: status {r['status']}")
+        roots.append(round(dt, 4))
+    out["seconds_root_bound"] = roots
+    out["total_s_root_bound"] = round(float(np.sum(roots)), 4)
+    out["speedup_bound_children_free"] = round(sum(secs["plain"]) / sum(roots), 3)
+    progress("time to proof: root bounds timed")
     if cpu_seeds:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O

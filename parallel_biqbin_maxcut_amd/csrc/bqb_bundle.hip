@@ -200,27 +200,32 @@ __device__ __forceinline__ int solve_lambda_reg(int k, const double *Q, const do
             const double q = Q[qrow + (j < k ? j : 0) * k];
             Ar[j] = (j == lane) ? -(-q - zl) : q;
         });
-        // right-looking Cholesky (wave_chol).  Column j goes to LDS at
-        // T[j*(K+1) + i]; lane j keeps 1/L_jj in rd.
+        // right-looking Cholesky (wave_chol).  The entries L_l2j of column
+        // j reach every lane by v_readlane (no LDS round trip between
+        // pivots); the column also goes to LDS at T[j*(K+1) + i] for the
+        // transpose below; lane j keeps 1/L_jj in rd.
+        // The pivots run without branches, so that the compiler can
+        // interleave them: pivots past k only touch padding rows and columns,
+        // and a non-positive real pivot is reported after the loop (the
+        // factor is not used then).
         bool ok = true;
         double rd = 0.0;
         static_for<0, K>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            if (j >= k || !ok) return;
             const double dj = lane_value(Ar[j], j);
-            if (!(dj > 0.0)) { ok = false; return; }
+            ok = ok & ((j >= k) | (dj > 0.0));
             const double ljj = sqrt(dj);
             const double rj = 1.0 / ljj;
             const double lij = (lane == j) ? ljj : Ar[j] * rj;
             Ar[j] = lij;
             if (lane == j) rd = rj;
             T[j * (K + 1) + lane] = lij;
-            __builtin_amdgcn_wave_barrier();
             static_for<j + 1, K>([&](auto lc) {
                 constexpr int l2 = decltype(lc)::value;
-                Ar[l2] -= lij * T[j * (K + 1) + l2];
+                Ar[l2] -= lij * lane_value(lij, l2);
             });
         });
+        __builtin_amdgcn_wave_barrier();
         if (!ok) return ERR_QP;
         // the factor's transpose: lane i gets row i of L' (L[j][i], j >= i)
         static_for<0, K>([&](auto jc) {

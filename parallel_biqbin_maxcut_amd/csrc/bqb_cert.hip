@@ -107,24 +107,30 @@ __device__ double certify(int n, int ld, const double *L, const double *y, const
                           double sum_gamma_ru, double *Zt, double *R, int ldr, double *rs, double *red)
 {
     const int tid = threadIdx.x;
-    // 1. Z~ and the row sums of the enclosure widths
+    // 1. Z~ and the enclosure widths, entry by entry over all threads (the
+    // gather lists make each entry a chain of dependent loads); the widths
+    // are parked in R (not used before step 2) and summed by rows in column
+    // order
+    for (int e = tid; e < n * n; e += CE) {
+        const int i = e % n, j = e / n;
+        const double l = L[i + (size_t)j * ld];
+        double zt, e1;
+        if (i == j) {
+            zt = y[i] - l;
+            e1 = fmax(bqb_sub_ru(bqb_sub_ru(y[i], l), zt), bqb_sub_ru(zt, bqb_sub_rd(y[i], l)));
+        } else {
+            double rn, lo, hi;
+            l0_entry(cs, l, i, j, rn, lo, hi);
+            zt = -rn;                                       // Z_ij = -L0_ij
+            e1 = fmax(bqb_sub_ru(hi, rn), bqb_sub_ru(rn, lo));
+        }
+        Zt[i + (size_t)j * ld] = zt;
+        R[i + (size_t)j * ldr] = e1;
+    }
+    __syncthreads();
     for (int i = tid; i < n; i += CE) {
         double acc = 0.0;
-        for (int j = 0; j < n; ++j) {
-            const double l = L[i + (size_t)j * ld];
-            double zt, e1;
-            if (i == j) {
-                zt = y[i] - l;
-                e1 = fmax(bqb_sub_ru(bqb_sub_ru(y[i], l), zt), bqb_sub_ru(zt, bqb_sub_rd(y[i], l)));
-            } else {
-                double rn, lo, hi;
-                l0_entry(cs, l, i, j, rn, lo, hi);
-                zt = -rn;                                   // Z_ij = -L0_ij
-                e1 = fmax(bqb_sub_ru(hi, rn), bqb_sub_ru(rn, lo));
-            }
-            Zt[i + (size_t)j * ld] = zt;
-            acc = bqb_add_ru(acc, e1);
-        }
+        for (int j = 0; j < n; ++j) acc = bqb_add_ru(acc, R[i + (size_t)j * ldr]);
         rs[i] = acc;
     }
     __syncthreads();
@@ -141,16 +147,17 @@ __device__ double certify(int n, int ld, const double *L, const double *y, const
     for (int att = 0; att < 4 && !ok; ++att) {
         delta = (att == 0) ? 0.0 : zmax * ((att == 1) ? 1e-12 : (att == 2) ? 1e-10 : 1e-8);
         double dsh = 0.0;
-        for (int i = tid; i < n; i += CE)
-            for (int j = 0; j <= i; ++j) {
-                double v = Zt[i + (size_t)j * ld];
-                if (i == j) {
-                    const double vd = v + delta;
-                    dsh = fmax(dsh, bqb_sub_ru(vd, v));
-                    v = vd;
-                }
-                R[i + (size_t)j * ldr] = v;
+        for (int e = tid; e < n * n; e += CE) {
+            const int i = e % n, j = e / n;
+            if (j > i) continue;
+            double v = Zt[i + (size_t)j * ld];
+            if (i == j) {
+                const double vd = v + delta;
+                dsh = fmax(dsh, bqb_sub_ru(vd, v));
+                v = vd;
             }
+            R[i + (size_t)j * ldr] = v;
+        }
         dshift = red_max(dsh, red);      // (its barriers also publish R)
         ok = true;
         for (int k = 0; k < n; ++k) {

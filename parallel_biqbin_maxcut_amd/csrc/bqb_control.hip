@@ -273,11 +273,16 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         if (threadIdx.x == 0) {
             int m = 0;
             if (s.active && !s.status) {
+                // the prune test reads the certified bound where bounds are
+                // certified; the give-up test (like every choice that only
+                // steers the work) reads f + fixedvalue, as the reference
                 const double bound = node_bound(s, d.certify);
                 sp->bound = bound;
-                if (d.certify) sp->basic_bound = bound;
                 if (bound < s.LB + 1.0) { sp->prune = 1; sp->active = 0; }
-                else if (P.use_diff && !s.is_root && bound > s.LB + s.diff + 1.0) { sp->giveup = 1; sp->active = 0; }
+                else if (P.use_diff && !s.is_root && s.f + s.fixedvalue > s.LB + s.diff + 1.0) {
+                    sp->giveup = 1;
+                    sp->active = 0;
+                }
                 else m = 1;
             } else sp->active = 0;
             sp->sep_tri = m;
@@ -312,7 +317,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             p.F[(size_t)node * d.max_bundle] = F0;
             sp->bundle = 1;
             sp->max_bundle_seen = 1;
-            sp->t = 0.5 * (s.bound - s.LB) / (m * s.viol3 * s.viol3);
+            sp->t = 0.5 * (s.f + s.fixedvalue - s.LB) / (m * s.viol3 * s.viol3);
             sp->cuts_changed = 1;
             sp->stage = BQB_STAGE_LOOP;
             // an inherited node evaluates its multipliers (PH_WARM, op_Bt,
@@ -369,7 +374,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
             if (s.active) {
                 int prune = s.prune;
                 if (!prune) prune = (s.bound < s.LB + 1.0) ? 1 : 0;
-                const double gap = s.bound - s.LB;
+                const double gap = s.f + s.fixedvalue - s.LB;   // steers only: uncertified
                 int giveup = 0;
                 if (s.count == P.triag_iter + P.pent_iter + P.hept_iter)
                     if (gap - 1.0 > (s.oldf - s.f) * (P.max_outer_iter - s.count)) giveup = 1;
@@ -416,7 +421,7 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
                 sp->done = 1;
                 sp->active = 0;
                 sp->bound = node_bound(s, d.certify);
-                if (s.is_root && !s.status) sp->diff = s.basic_bound - sp->bound;
+                if (s.is_root && !s.status) sp->diff = s.basic_bound - (s.f + s.fixedvalue);
                 p.mask[node] = 0;
             }
             return;

@@ -257,6 +257,38 @@ def test_certified_fp64_bound_tracks_plain_bound():
         assert 0.0 <= d <= 1e-7 * abs(plain["bound"][b]), (b, cert["bound"][b], plain["bound"][b])
 
 
+def test_certified_bounds_only_change_the_prune_tests():
+    """Certification replaces the value the prune tests read and nothing
+    else: the bundle's initial t (bounding.c:133), the give-up tests and
+    the root's diff keep reading f + fixedvalue, so a root bound follows the
+    plain trajectory bit for bit (every round's f), and a branch and bound
+    visits the same nodes in the same launch rounds."""
+    n0 = 60
+    SP = api.maxcut_L(I.gnp_adj(n0, 0.5, 2, weights="unit"))
+    P = api.make_params(include_Pent=0, include_Hepta=0)
+    P.TriIneq = 10 * n0                       # src/process_input.c:135-136
+    z = np.zeros(n0 - 1, np.int32)
+    runs = []
+    for cert in (0, 1):
+        ctx = api.BoundContext(SP, params=P, max_nodes=1)
+        ctx.set_precision(slices=0, certify=cert)
+        ctx.capture(True)
+        ctx.push(z, z, LB=0.0, diff=0.0, is_root=1, seed=2020, tag=0)
+        while ctx.busy():
+            ctx.advance()
+        runs.append((ctx.pop(), ctx.trajectories()[0]))
+        ctx.close()
+    (r0, t0), (r1, t1) = runs
+    assert r0["status"] == 0 and r1["status"] == 0 and t0["rounds"] > 5
+    assert np.array_equal(t0["round_f"], t1["round_f"])
+    assert np.array_equal(r0["stats"], r1["stats"]) and r0["diff"] == r1["diff"]
+    assert 0.0 <= r1["bound"] - r0["bound"] <= 1e-7 * abs(r0["bound"])
+    plain, _ = api.solve(SP, P, seed=2020, batch=64, warm=True)
+    cert, _ = api.solve(SP, P, seed=2020, batch=64, warm=True, certify=1)
+    assert plain.status == 0 and cert.status == 0 and cert.best == plain.best
+    assert (cert.nodes, cert.batches) == (plain.nodes, plain.batches)
+
+
 @pytest.mark.parametrize("name", ["g05_60.0", "g05_80.0"])
 def test_mixed_certified_bab_proves_fixture_optimum(golden, name):
     path = os.path.join(golden, "rudy", name)

@@ -392,7 +392,8 @@ void bqb_ctx_trajectory_clear(bqb_ctx *c);
  * the basic bound / of the bundle center of the nodes whose center moved
  * since their last certificate (mask phase 16), 112/113 learned model on the
  * cuts of the first / of a round's separation, 114 certificate of a
- * warm-started center, 115 upload the nodes pushed since the last advance
+ * warm-started center right after its rebuild (the whole bound leaves it to
+ * the round end, 111), 115 upload the nodes pushed since the last advance
  * and run only their basic sequence), run the whole bound,
  * read results, and copy one node's
  * device buffer (ids documented in bqb_sdpbound.c) to or from the host. */

@@ -244,7 +244,8 @@ class NodeState(C.Structure):
         "n", "status", "active", "prune", "giveup", "done", "count", "bdl_iter", "NIneq",
         "NPentIneq", "NHeptaIneq", "bundle", "sep_tri", "sep_pent", "sep_hept", "heur_idx",
         "improved", "ipm_calls", "ipm_iters", "max_cuts", "max_bundle_seen", "is_root", "serious",
-        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags", "warm", "cert_stale")] + [(f, C.c_double) for f in (
+        "cuts_changed", "off_pent", "off_hept", "stage", "bdl_left", "rphase", "flags", "warm", "cert_stale",
+        "nsurv_tri", "nsurv_pent", "nsurv_hept")] + [(f, C.c_double) for f in (
         "fixedvalue", "f", "f_test", "t", "bound", "basic_bound", "oldf", "LB", "diff", "viol3",
         "viol5", "gap", "del_", "lmax", "f_cert")] + [("seed", C.c_uint64)]
 

@@ -48,6 +48,8 @@ typedef struct NodeState {
                            bundle center is evaluated at them (until done) */
     int cert_stale;     /* the center moved (serious step) since f_cert was
                            computed; its y is in ycen */
+    int nsurv_tri, nsurv_pent, nsurv_hept;   /* with a learned model: cuts of
+                           each list before the last separation's new ones */
     double fixedvalue, f, f_test, t, bound, basic_bound, oldf;
     double LB, diff, viol3, viol5, gap, del, lmax;
     double f_cert;      /* certified upper bound of f (when certifying) */

@@ -259,14 +259,15 @@ __global__ __launch_bounds__(CE) void cert_ipm_kernel(const int *nvec, const dou
 
 // ---- the node bound: certificate of the point just evaluated ----
 // mode 0: basic bound (no cuts; phi of the IPM on L, y in ytest);
-// mode 1: the center of a node whose center moved (serious steps) since its
-// last certificate, at the end of its round (gamma the center, y of the
-// center's evaluation saved in ycen; the cut lists and gather lists are
-// those of the bundle iterations that moved it), the value the round-end
-// prune test reads -- the same value a certificate after every serious step
-// would leave there;
-// mode 2: after the rebuild of a round that made the warm-started center
-// gamma the new f (y of the IPM on L - B'gamma).
+// mode 1: the center of a node whose center moved (serious steps, or a
+// rebuild that made warm-started multipliers the center) since its last
+// certificate, at the end of its round (gamma the center, y of the center's
+// evaluation saved in ycen; the cut lists and gather lists are those of the
+// evaluations that moved it), the value the round-end prune test reads --
+// the same value a certificate after every move would leave there;
+// mode 2: right after a rebuild that made the warm-started center gamma the
+// new f (y of the IPM on L - B'gamma; the launch trains leave such a center
+// to mode 1).
 // The certified value replaces NodeState.f_cert.
 template <int NL>
 __global__ __launch_bounds__(CE) void cert_node_kernel(BatchDims d, BatchPtrs p, int mode)

@@ -486,12 +486,34 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
         const bool keep_center = !s.warm || (fnew < s.f && !p.ipm_status[node]);
         if (!keep_center)
             for (int c = threadIdx.x; c < m; c += CT) gam[c] = 0.0;
+        // The learned warm start of the new cuts moves the center to them
+        // only where it lowers f; otherwise the new cuts start at 0 as in the
+        // reference (cutting_planes.c:201, 392, 588) and f stays (bounding.c:
+        // 278 extends the bundle without a new center).
+        const bool learned_keep = !d.learned || s.warm || (fnew < s.f && !p.ipm_status[node]);
+        if (!learned_keep) {
+            const int a0 = s.nsurv_tri, a1 = s.NIneq + s.nsurv_pent, a2 = s.NIneq + s.NPentIneq + s.nsurv_hept;
+            for (int c = threadIdx.x; c < m; c += CT) {
+                const bool fresh = (c >= a0 && c < s.NIneq) || (c >= a1 && c < s.NIneq + s.NPentIneq) || c >= a2;
+                if (fresh) gam[c] = 0.0;
+            }
+        }
+        // a center moved here (warm-started multipliers) is certified at the
+        // end of the round like one moved by a serious step: its y is kept,
+        // and until then the last certificate stays a valid bound
+        const bool moved = d.certify && !p.ipm_status[node] && (s.warm ? keep_center : d.learned && learned_keep);
+        if (moved) {
+            const double *yt = p.ytest + (size_t)node * d.ld;
+            double *yc = p.ycen + (size_t)node * d.ld;
+            for (int i = threadIdx.x; i < s.n; i += CT) yc[i] = yt[i];
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             const int failed = p.ipm_status[node];
             sp->ipm_calls += 1;
             sp->ipm_iters += p.ipm_iters[node];
             sp->rphase = 0;
+            if (moved) sp->cert_stale = 1;
             if (s.warm) {
                 // a failed evaluation of inherited multipliers adds nothing
                 // and leaves the node on its reference path (gamma = 0)
@@ -503,10 +525,9 @@ __global__ __launch_bounds__(CT) void control_kernel(BatchDims d, BatchPtrs p, i
                     if (sp->bundle > s.max_bundle_seen) sp->max_bundle_seen = sp->bundle;
                 }
                 sp->warm = 0;
-                // a certificate of the new center follows only where it moved
                 p.mask[node] = keep_center ? 1 : 0;
             } else {
-                if (d.learned) sp->f = fnew;
+                if (d.learned && learned_keep) sp->f = fnew;
                 p.F[(size_t)node * d.max_bundle + s.bundle] = Fb;
                 sp->bundle = s.bundle + 1;
                 if (sp->bundle > s.max_bundle_seen) sp->max_bundle_seen = sp->bundle;

@@ -336,7 +336,9 @@ __global__ __launch_bounds__(LEARN_T) void learn_cuts_kernel(BatchDims d, BatchP
     cx.lscale = (float)(n > 1 ? lsum / ((double)n * (n - 1)) : 1.0) + 1e-12f;
     cx.round = (float)s.count;
     cx.cuts = (float)m;
-    cx.gap = (float)((s.bound - s.LB) / (fabs(s.bound) + 1.0));
+    // the uncertified bound (certification changes only the prune tests)
+    const double plain_bound = s.f + s.fixedvalue;
+    cx.gap = (float)((plain_bound - s.LB) / (fabs(plain_bound) + 1.0));
     // the new cuts, in cut order (ordered compaction of y == 0)
     int *idx = la.idx + (size_t)node * d.max_m;
     unsigned char *keep = la.keep + (size_t)node * d.max_m;
@@ -359,6 +361,21 @@ __global__ __launch_bounds__(LEARN_T) void learn_cuts_kernel(BatchDims d, BatchP
         if (isnew) idx[nnew + s_cnt[threadIdx.x]] = c;
         nnew += s_nnew;
         __syncthreads();
+    }
+    // the new cuts follow the survivors of each list's purge (and stay
+    // behind them when some are dropped below): the rebuild that evaluates
+    // the warm start reads where they begin
+    if (threadIdx.x == 0) {
+        int ft = mt, fp = mt + mp, fh = m;
+        for (int q = nnew - 1; q >= 0; --q) {
+            const int c = idx[q];
+            if (c < mt) ft = c;
+            else if (c < mt + mp) fp = c;
+            else fh = c;
+        }
+        sp->nsurv_tri = ft;
+        sp->nsurv_pent = fp - mt;
+        sp->nsurv_hept = fh - mt - mp;
     }
     if (la.rec_count) {
         // record a quarter of the new cuts (by hash of the cut)

@@ -930,8 +930,7 @@ static int basic_train(bqb_ctx *c)
         RUN(PH_WARM, 0);
         RUN(107, 0);         /* L0 = L - B'(gamma) */
         RUN(101, 0);         /* IPM: L0 -> X_test */
-        RUN(PH_REBUILD, 1);
-        if (c->d.certify) RUN(114, 0);   /* certify the warm-started center */
+        RUN(PH_REBUILD, 1);  /* a kept center is certified at its first round end */
     }
     return 0;
 }
@@ -970,8 +969,7 @@ static int round_train(bqb_ctx *c, int nb, int kmax)
     RUN(106, 0);             /* new gather lists */
     RUN(107, 0);             /* L0 = L - B'(gamma) */
     RUN(101, 0);             /* IPM: L0 -> X_test */
-    RUN(PH_REBUILD, 0);
-    if (c->d.learned && c->d.certify) RUN(114, 0);   /* certify the warm-started center */
+    RUN(PH_REBUILD, 0);      /* a moved center is certified at the round end */
     return 0;
 }
 

@@ -141,6 +141,75 @@ def test_model_in_bound_gives_valid_bounds(drop):
         mlp.close()
 
 
+def _round_to_rebuild(ctx, nb):
+    """One outer round of a one-slot context, phase by phase as the launch
+    train runs it, up to (not including) the rebuild."""
+    PH = api.PH
+    ctx.run(PH["OUTER_BEGIN"])
+    for b in range(nb):
+        ctx.run(PH["BUNDLE_NEXT"], b)
+        ctx.run(PH["QP"], 0)
+        ctx.run(PH["OPBT"], 1)
+        ctx.run(PH["IPM_L0"])
+        ctx.run(PH["STEP"])
+    for ph in ("OUTER_A", "HEUR", "OUTER_B", "TRISEP", "SEP_PENT", "PENT", "SEP_HEPT", "HEPT"):
+        ctx.run(PH[ph])
+    ctx.run(113)                       # the model on the round's new cuts
+    for ph, arg in (("OUTER_C", 0), ("CSR", 0), ("OPBT", 0), ("IPM_L0", 0)):
+        ctx.run(PH[ph], arg)
+
+
+@pytest.mark.parametrize("kind", ["trained", "huge"])
+def test_learned_warm_start_kept_only_where_it_lowers_f(kind):
+    """The rebuild after a separation evaluates the model's multipliers of
+    the new cuts; the center moves to them only where that lowers f.  A
+    model biased to predict multipliers near 20 for every cut raises f:
+    the center and f stay and the new cuts start at 0, as in the reference
+    (bounding.c:278 extends the bundle only).  The trained model never
+    raises f at a rebuild either."""
+    import os
+    path = os.path.join(os.path.dirname(learn.__file__), "models", "cut_model.npz")
+    prm = np.array(np.load(path, allow_pickle=False)["params"], dtype=np.float32)
+    mlp = learn.MLP(params=prm)
+    n0 = 40
+    SP = api.maxcut_L(I.gnp_adj(n0, 0.5, 3, weights="unit"))
+    P = api.make_params(include_Pent=0, include_Hepta=0)
+    P.TriIneq = 10 * n0
+    ctx = api.BoundContext(SP, params=P, max_nodes=1)
+    try:
+        api.N.check(api.N.lib().bqb_ctx_set_model(ctx.h, mlp.h, 0.0))
+        z = np.zeros(n0 - 1, np.int32)
+        slot = ctx.push(z, z, LB=0.0, diff=0.0, is_root=1, seed=5, tag=0)
+        ctx.run(api.PH["BASIC_TRAIN"])
+        if kind == "huge":
+            # from the first round end on (the first separation's cuts stay
+            # as the trained model set them, so that later rounds add cuts)
+            b4 = learn.NPARAM - 16         # the output layer's 16 biases (rows 0 and 1 used)
+            prm[b4] = prm[b4 + 1] = 100.0  # magnitude at its cap, P(active) = 1
+            mlp.set_params(prm)
+        for r in range(2):
+            st = ctx.state(slot)
+            assert st.status == 0 and st.active == 1 and st.rphase == 0
+            _round_to_rebuild(ctx, P.init_bundle_iter if r == 0 else P.init_bundle_iter + 1)
+            before = ctx.state(slot)
+            assert before.sep_tri == 1 and before.nsurv_tri < before.NIneq     # new cuts exist
+            m = before.NIneq
+            g0 = ctx.get("gamma", slot, np.float64, m)
+            ctx.run(api.PH["REBUILD"])
+            after = ctx.state(slot)
+            g1 = ctx.get("gamma", slot, np.float64, m)
+            assert after.status == 0 and after.f <= before.f
+            assert np.array_equal(g1[:before.nsurv_tri], g0[:before.nsurv_tri])   # survivors untouched
+            if kind == "huge":
+                assert np.all(g0[before.nsurv_tri:] > 1.0)                         # predictions made
+                assert after.f == before.f and np.all(g1[before.nsurv_tri:] == 0.0)
+            elif after.f < before.f:
+                assert np.array_equal(g1, g0)                                      # the warm start kept
+    finally:
+        ctx.close()
+        mlp.close()
+
+
 def test_generate_and_train_from_solver_data():
     """The whole loop on a small scale: solver-generated pairs from 40
     G(80, p) graphs, trained on the GPU; the held-out loss falls below the

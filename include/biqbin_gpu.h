@@ -91,6 +91,11 @@ typedef struct BiqBinParameters {
 /* ---- library management ---------------------------------------------- */
 int bqb_init(int device);                 /* select the HIP device (default 0) */
 const char *bqb_last_error(void);
+/* Sizes in bytes of the structures a binding mirrors, in this order:
+ * BiqBinParameters, bqb_solve_options, bqb_solve_stats, bqb_trajectory and
+ * the node state that bqb_ctx_download copies out (buffer id 0).  Writes
+ * min(count, 5) entries and returns 5. */
+int bqb_abi_sizes(int *sizes, int count);
 
 /* Binary layout of the reference's Problem and BabNode
  * (src/biqbin_cpp_api.h:27-63, NMAX = 1024). */

@@ -413,6 +413,8 @@ def _declare_ctx(L):
     L.bqb_ctx_ld.restype = C.c_int
     L.bqb_ctx_last_ms.argtypes = [vp]
     L.bqb_ctx_last_ms.restype = C.c_double
+    L.bqb_abi_sizes.argtypes = [N.iptr, C.c_int]
+    L.bqb_abi_sizes.restype = C.c_int
     u64p = C.POINTER(C.c_uint64)
     L.bqb_bound_batch.argtypes = [vp, C.c_int, N.iptr, N.iptr, N.dptr, N.dptr, N.iptr, u64p, N.dptr, N.dptr,
                                   N.iptr, N.dptr, N.iptr, N.dptr, N.iptr, N.iptr]

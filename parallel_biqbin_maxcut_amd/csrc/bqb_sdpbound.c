@@ -424,6 +424,14 @@ size_t bqb_ctx_bytes(const bqb_ctx *c) { return c->arena_bytes; }
 double bqb_ctx_last_ms(const bqb_ctx *c) { return c->last_ms; }
 int bqb_ctx_ld(const bqb_ctx *c) { return c->d.ld; }
 
+int bqb_abi_sizes(int *sizes, int count)
+{
+    const int z[5] = {(int)sizeof(BiqBinParameters), (int)sizeof(bqb_solve_options), (int)sizeof(bqb_solve_stats),
+                      (int)sizeof(bqb_trajectory), (int)sizeof(NodeState)};
+    for (int i = 0; i < count && i < 5; ++i) sizes[i] = z[i];
+    return 5;
+}
+
 static int ipm_launch(bqb_ctx *c, const double *L, double *X)
 {
     IpmBatchArgs a;

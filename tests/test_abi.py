@@ -34,6 +34,18 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_struct_mirrors_match_the_library():
+    """The ctypes mirrors of api.py have the C structures' sizes (a field
+    added on one side only shifts every later field)."""
+    import ctypes
+    import numpy as np
+    from parallel_biqbin_maxcut_amd import _native as N, api
+    sizes = np.zeros(5, np.int32)
+    assert N.lib().bqb_abi_sizes(sizes.ctypes.data_as(N.iptr), 5) == 5
+    mirrors = (api.BiqBinParameters, api.SolveOptions, api.SolveStats, api.Trajectory, api.NodeState)
+    assert [ctypes.sizeof(m) for m in mirrors] == sizes.tolist()
+
+
 def test_library_loads_without_gpu():
     from parallel_biqbin_maxcut_amd import _native as N
     L = N.lib()

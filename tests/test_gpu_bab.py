@@ -98,6 +98,19 @@ def test_cli_reports_the_fixture(golden, tmp_path):
     assert abs(rb - 536.95) <= ROOT_BOUND_REL * 536.95 + 0.005
 
 
+@pytest.mark.parametrize("opts", [["-w", "-p", "auto"], ["-w", "-p", "4", "-g"], ["-c"]])
+def test_cli_options_prove_the_fixture(golden, opts):
+    """The CLI's solver forms (inherited cuts, automatic or bf16 precision
+    with certified bounds, graphs) prove the same optimum."""
+    exe = os.path.join(PKG, "biqbin_gpu")
+    r = subprocess.run([exe] + opts + [os.path.join(golden, "rudy", "g05_60.0"), os.path.join(golden, "params")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Maximum value = 536" in r.stdout
+    bad = subprocess.run([exe, "-p", "9", "x", "y"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "Usage" in bad.stderr
+
+
 def test_dropin_evaluate_root_node(golden):
     """Evaluate() with the reference's signature on the root node: same
     bound as the batch entry point with the drop-in's stream key."""

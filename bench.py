@@ -115,6 +115,9 @@ PROOF_FORMS = (
     ("batched_inherited_mixed_learned", dict(batch=64, warm=True, slices=-1, learned=True)),
     # the IPM products forced onto the bf16 matrix cores (4 slices)
     ("batched_inherited_bf16", dict(batch=64, warm=True, slices=4)),
+    # every bundle-step IPM warm-started from the node's previous solve
+    ("batched_inherited_ipmwarm", dict(batch=64, warm=True, ipm_warm=True)),
+    ("batched_inherited_mixed_learned_ipmwarm", dict(batch=64, warm=True, slices=-1, learned=True, ipm_warm=True)),
 )
 
 
@@ -194,14 +197,16 @@ def time_to_proof(api, I, seeds, cpu_seeds):
     # starts with it and no child can start before it ends, so the plain
     # total over the roots' total bounds any form's speedup, even one whose
     # children cost nothing.
-    roots = []
-    for L in Ls:
+    roots, roots_w = [], []
+    for L, ipm_warm in [(L, w) for L in Ls for w in (False, True)]:
         B = L.shape[0] - 1
         Pr = api.make_params(include_Pent=0, include_Hepta=0)
         Pr.TriIneq = 10 * L.shape[0]               # src/process_input.c:135-136
         z = np.zeros(B, np.int32)
         for _ in range(2):                          # the first is untimed
             ctx = api.BoundContext(L, params=Pr, max_nodes=1)
+            if ipm_warm:
+                ctx.set_ipm_warm(True)
             t0 = time.perf_counter()
             ctx.push(z, z, LB=0.0, diff=0.0, is_root=1, seed=2020, tag=0)
             while ctx.busy():
@@ -211,10 +216,14 @@ def time_to_proof(api, I, seeds, cpu_seeds):
             ctx.close()
         if r["status"]:
             raise SystemExit(f"time-to-proof root bound: status {r['status']}")
-        roots.append(round(dt, 4))
+        (roots_w if ipm_warm else roots).append(round(dt, 4))
     out["seconds_root_bound"] = roots
     out["total_s_root_bound"] = round(float(np.sum(roots)), 4)
     out["speedup_bound_children_free"] = round(sum(secs["plain"]) / sum(roots), 3)
+    # the same with warm-started bundle IPMs (the *_ipmwarm forms' root)
+    out["seconds_root_bound_ipm_warm"] = roots_w
+    out["total_s_root_bound_ipm_warm"] = round(float(np.sum(roots_w)), 4)
+    out["speedup_bound_children_free_ipm_warm"] = round(sum(secs["plain"]) / sum(roots_w), 3)
     progress("time to proof: root bounds timed")
     if cpu_seeds:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -201,7 +201,9 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
  * reference).  warm = 1: every child node inherits its parent's final cuts
  * whose multiplier is >= 1e-5 and that do not contain the newly fixed
  * variable, with their multipliers (bqb_ctx_set_warm).  graphs: 1 launch
- * trains as HIP graphs, 0 eager, -1 the library default. */
+ * trains as HIP graphs, 0 eager, -1 the library default.  ipm_warm = 1:
+ * every bundle-step IPM starts from the slot's previous solve
+ * (bqb_ctx_set_ipm_warm). */
 typedef struct bqb_solve_options {
     int batch;            /* node slots (>= 1) */
     int max_bundle;       /* bundle capacity per node (<= 0: 64) */
@@ -210,6 +212,7 @@ typedef struct bqb_solve_options {
     float drop;
     int warm;
     int graphs;
+    int ipm_warm;
 } bqb_solve_options;
 int bqb_solve_opt(int n0, const double *SP_L, const BiqBinParameters *params, uint64_t seed,
                   const bqb_solve_options *opt, int *sol, bqb_solve_stats *stats);
@@ -511,6 +514,18 @@ int bqb_ctx_set_graphs(bqb_ctx *c, int on);
  * popped node's final cut set and multipliers (empty if it was pruned or
  * had none); the pointers stay valid until the next pop. */
 int bqb_ctx_set_warm(bqb_ctx *c, int on);
+/* Warm-started bundle IPMs (no counterpart in the reference, whose
+ * ipm_mc_pk.c:65-99 starts every solve at X = I, y = 1.1 + column sums of
+ * |L|).  With on = 1, each IPM of a bundle step (L - B'(gamma) -> X) starts
+ * from the previous one of the same node: X0 = 0.95 X + 0.05 I and
+ * y0_i = y_i + dL_ii + sum_{j != i} |dL_ij| + 0.01, dL the change of the
+ * objective, which keeps the dual slack positive definite; if rounding
+ * defeats that, the solve restarts at the reference's point.  Bounds stay
+ * valid (every iterate's y is dual feasible) and agree with the cold solves'
+ * within the IPM's 1e-2 gap; the trajectory, and so the node count, may
+ * differ.  Register-tile solves only (n up to the largest register tile);
+ * larger nodes start cold.  Allocates max_nodes * (ld^2 doubles + 1 int). */
+int bqb_ctx_set_ipm_warm(bqb_ctx *c, int on);
 int bqb_ctx_push_cuts(bqb_ctx *c, int slot, int ntri, const int *tri, int npent, const int *pent, int nhept,
                       const int *hept, const double *gamma);
 int bqb_ctx_last_cuts(const bqb_ctx *c, int *ntri, const int **tri, int *npent, const int **pent, int *nhept,

@@ -282,7 +282,8 @@ def _arr(ptr, count, dtype, shape=None):
 class SolveOptions(C.Structure):
     """bqb_solve_options"""
     _fields_ = [("batch", C.c_int), ("max_bundle", C.c_int), ("slices", C.c_int), ("certify", C.c_int),
-                ("model", C.c_void_p), ("drop", C.c_float), ("warm", C.c_int), ("graphs", C.c_int)]
+                ("model", C.c_void_p), ("drop", C.c_float), ("warm", C.c_int), ("graphs", C.c_int),
+                ("ipm_warm", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -385,19 +386,20 @@ def evaluate_solution(SP_L, sol):
 
 
 def solve(SP_L, params=None, seed=2020, batch=64, max_bundle=0, slices=0, certify=0, model=None, drop=0.0,
-          graphs=None, warm=False):
+          graphs=None, warm=False, ipm_warm=False):
     """Branch and bound on the GPU (bqb_solve_opt).  Returns (stats,
     solution) with the solution over the first n0-1 vertices (the last is on
     side 0).  graphs: launch trains as HIP graphs (None: the library's
     default).  warm: children inherit their parent's final cuts and
-    multipliers."""
+    multipliers.  ipm_warm: every bundle-step IPM starts from the node's
+    previous one."""
     SP_L = np.asfortranarray(SP_L, dtype=np.float64)
     n0 = SP_L.shape[0]
     p = params if params is not None else make_params()
     sol = np.zeros(n0 - 1, dtype=np.int32)
     st = SolveStats()
     o = SolveOptions(batch, max_bundle, slices, certify, model.h if model is not None else None, drop,
-                     1 if warm else 0, -1 if graphs is None else (1 if graphs else 0))
+                     1 if warm else 0, -1 if graphs is None else (1 if graphs else 0), 1 if ipm_warm else 0)
     N.check(N.lib().bqb_solve_opt(n0, _d(SP_L), C.byref(p), seed, C.byref(o), _i(sol), C.byref(st)))
     return st, sol
 
@@ -434,6 +436,7 @@ def _declare_ctx(L):
     L.bqb_ctx_busy.argtypes = [vp]
     L.bqb_ctx_capacity.argtypes = [vp]
     L.bqb_ctx_set_warm.argtypes = [vp, C.c_int]
+    L.bqb_ctx_set_ipm_warm.argtypes = [vp, C.c_int]
     L.bqb_ctx_push_cuts.argtypes = [vp, C.c_int, C.c_int, N.iptr, C.c_int, N.iptr, C.c_int, N.iptr, N.dptr]
     ipp = C.POINTER(N.iptr)
     L.bqb_ctx_last_cuts.argtypes = [vp, N.iptr, ipp, N.iptr, ipp, N.iptr, ipp, C.POINTER(N.dptr)]
@@ -669,6 +672,11 @@ class BoundContext:
     def set_warm(self, on=True):
         for h in self.handles:
             N.check(N.lib().bqb_ctx_set_warm(h, 1 if on else 0))
+
+    def set_ipm_warm(self, on=True):
+        """bqb_ctx_set_ipm_warm: bundle-step IPMs start from the slot's last."""
+        for h in self.handles:
+            N.check(N.lib().bqb_ctx_set_ipm_warm(h, 1 if on else 0))
 
     def push(self, xfixed, solX, LB=0.0, diff=0.0, is_root=0, seed=0, tag=0):
         """bqb_ctx_push of one node; returns its slot."""

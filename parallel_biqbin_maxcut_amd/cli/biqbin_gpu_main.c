@@ -13,6 +13,7 @@
  *               certified bounds
  *   -c          certified bounds with FP64 products
  *   -g          launch trains as HIP graphs
+ *   -i          bundle-step IPMs start from the node's previous solve
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -24,16 +25,16 @@
 
 static int usage(const char *prog)
 {
-    fprintf(stderr, "Usage: %s [-w] [-p auto|2|3|4] [-c] [-g] file.rudy file.params [nodes per batch]\n", prog);
+    fprintf(stderr, "Usage: %s [-w] [-p auto|2|3|4] [-c] [-g] [-i] file.rudy file.params [nodes per batch]\n", prog);
     return 1;
 }
 
 int main(int argc, char **argv)
 {
     bqb_solve_options o = {.batch = 64, .max_bundle = 0, .slices = 0, .certify = 0, .model = NULL,
-                           .drop = 0.f, .warm = 0, .graphs = -1};
+                           .drop = 0.f, .warm = 0, .graphs = -1, .ipm_warm = 0};
     int opt;
-    while ((opt = getopt(argc, argv, "wp:cg")) != -1) {
+    while ((opt = getopt(argc, argv, "wp:cgi")) != -1) {
         switch (opt) {
         case 'w': o.warm = 1; break;
         case 'p':
@@ -45,6 +46,7 @@ int main(int argc, char **argv)
             break;
         case 'c': o.certify = 1; break;
         case 'g': o.graphs = 1; break;
+        case 'i': o.ipm_warm = 1; break;
         default: return usage(argv[0]);
         }
     }

@@ -389,7 +389,7 @@ int bqb_solve_ex(int n0, const double *SP_L, const BiqBinParameters *params, uin
                  int max_bundle, int slices, int certify, const bqb_mlp *model, float drop, int *sol,
                  bqb_solve_stats *out)
 {
-    const bqb_solve_options o = {batch, max_bundle, slices, certify, model, drop, 0, -1};
+    const bqb_solve_options o = {batch, max_bundle, slices, certify, model, drop, 0, -1, 0};
     return bqb_solve_opt(n0, SP_L, params, seed, &o, sol, out);
 }
 
@@ -421,7 +421,8 @@ int bqb_solve_opt(int n0, const double *SP_L, const BiqBinParameters *params, ui
     if (!ctx) return -1;
     const int warm = opt->warm;
     if (bqb_ctx_set_precision(ctx, opt->slices, opt->certify) || bqb_ctx_set_model(ctx, opt->model, opt->drop) ||
-        (opt->graphs >= 0 && bqb_ctx_set_graphs(ctx, opt->graphs)) || (warm && bqb_ctx_set_warm(ctx, 1))) {
+        (opt->graphs >= 0 && bqb_ctx_set_graphs(ctx, opt->graphs)) || (warm && bqb_ctx_set_warm(ctx, 1)) ||
+        (opt->ipm_warm && bqb_ctx_set_ipm_warm(ctx, 1))) {
         bqb_ctx_destroy(ctx);
         return -1;
     }

@@ -659,6 +659,7 @@ __device__ void blk_solve(const IpmBatchArgs &args, const int node, const int n,
     }
     if (args.y_out)
         for (int i = t.tid; i < n; i += NT) args.y_out[(size_t)node * args.ystride + i] = S.y[i];
+    if (args.warm_ok && t.tid == 0) args.warm_ok[node] = 0;     // starts cold
 }
 
 template <bool MIX>
@@ -669,7 +670,11 @@ __global__ __launch_bounds__(NT, 1) void ipm_blk_kernel(IpmBatchArgs args)
     if (args.active && !args.active[node]) return;
     const int n = args.n[node];
     if (n < 1 || n > NB || n > args.ld || args.wslabs < (MIX ? 6 : 4)) {
-        if (threadIdx.x == 0) { args.status[node] = ERR_SIZE; args.iters[node] = 0; }
+        if (threadIdx.x == 0) {
+            args.status[node] = ERR_SIZE;
+            args.iters[node] = 0;
+            if (args.warm_ok) args.warm_ok[node] = 0;
+        }
         return;
     }
     Tid t;

@@ -55,6 +55,13 @@ typedef struct IpmBatchArgs {
     int mix_slices;        /* 0: FP64 products; 2..4: products on bf16 matrix cores with this many slices */
     unsigned long long *flops_mx;   /* optional (mixed): += bf16 matrix-core flops, 2 n^3 per slice pair */
     int wslabs;            /* work slabs per node in `work` (the blocked solve needs 4) */
+    /* warm start from the slot's last solve (optional; needs y_out):
+     * warm_mode 0 clears warm_ok; 1 starts from X, y_out and lprev where
+     * warm_ok is set, keeps L in lprev and sets warm_ok to the solve's
+     * success.  The blocked solve always starts cold and clears warm_ok. */
+    double *lprev;         /* [nodes*ld*ld] the L of the slot's last solve */
+    int *warm_ok;          /* [nodes] */
+    int warm_mode;
 } IpmBatchArgs;
 
 /* Algorithmic FP64 flops of one IPM solve of size n: per iteration two SPD

@@ -106,6 +106,9 @@ struct bqb_ctx {
     float *d_rec_feat;         /* [rec_cap][16] */
     int *d_rec_key;            /* [rec_cap][12] */
     int *d_lidx;               /* [max_nodes][max_m] */
+    double *d_lprev;           /* [max_nodes][ld*ld] the L of each slot's last bundle IPM (warm IPM) */
+    int *d_warm_ok;            /* [max_nodes] */
+    int ipm_warm;
     unsigned char *d_lkeep;    /* [max_nodes][max_m] */
     /* host heuristic replacing the device GW (bqb_ctx_set_heuristic) */
     bqb_heuristic_fn heur_fn;
@@ -411,6 +414,7 @@ void bqb_ctx_destroy(bqb_ctx *c)
     graphs_clear(c);
     if (c->d_rec_count) hipFree(c->d_rec_count);
     if (c->d_lidx) hipFree(c->d_lidx);
+    if (c->d_lprev) hipFree(c->d_lprev);
     free_events(c);
     free_host(c);
     hipFree(c->arena);
@@ -448,9 +452,14 @@ static int ipm_launch(bqb_ctx *c, const double *L, double *X)
     a.active = c->p.mask;
     a.mix_slices = c->d.mix_slices;
     a.wslabs = BQB_IPM_WORK_SLABS(1);     /* the arena's stride per slot */
-    if (c->d.certify) {            /* the certificate needs y */
+    if (c->d.certify || c->ipm_warm) {   /* the certificate and the warm start need y */
         a.y_out = c->p.ytest;
         a.ystride = c->d.ld;
+    }
+    if (c->ipm_warm) {             /* bundle solves (L0 -> Xt) start from the slot's last */
+        a.lprev = c->d_lprev;
+        a.warm_ok = c->d_warm_ok;
+        a.warm_mode = (X == c->p.Xt) ? 1 : 0;
     }
     const int prof = c->profile_ipm && c->pev_n + 2 <= c->pev_cap;
     if (prof) {
@@ -674,6 +683,24 @@ int bqb_ctx_set_warm(bqb_ctx *c, int on)
     }
     graphs_clear(c);           /* the basic train changes */
     c->warm = on ? 1 : 0;
+    return 0;
+}
+
+int bqb_ctx_set_ipm_warm(bqb_ctx *c, int on)
+{
+    if (bqb_ctx_busy(c) || c->nload) { bqb_set_error("set_ipm_warm: nodes in flight"); return -1; }
+    if (on && !c->d_lprev) {
+        const size_t slab = (size_t)c->d.ld * c->d.ld, N = (size_t)c->max_nodes;
+        if (bqb_check(hipMalloc((void **)&c->d_lprev, sizeof(double) * N * slab + sizeof(int) * N),
+                      "hipMalloc warm IPM"))
+            return -1;
+        c->d_warm_ok = (int *)(c->d_lprev + N * slab);
+    }
+    /* no slot holds a usable last solve yet */
+    if (on && bqb_check(hipMemsetAsync(c->d_warm_ok, 0, sizeof(int) * c->max_nodes, c->stream), "hipMemset warm"))
+        return -1;
+    graphs_clear(c);           /* the IPM launches change */
+    c->ipm_warm = on ? 1 : 0;
     return 0;
 }
 

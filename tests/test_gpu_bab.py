@@ -98,10 +98,10 @@ def test_cli_reports_the_fixture(golden, tmp_path):
     assert abs(rb - 536.95) <= ROOT_BOUND_REL * 536.95 + 0.005
 
 
-@pytest.mark.parametrize("opts", [["-w", "-p", "auto"], ["-w", "-p", "4", "-g"], ["-c"]])
+@pytest.mark.parametrize("opts", [["-w", "-p", "auto"], ["-w", "-p", "4", "-g"], ["-c"], ["-w", "-i"]])
 def test_cli_options_prove_the_fixture(golden, opts):
     """The CLI's solver forms (inherited cuts, automatic or bf16 precision
-    with certified bounds, graphs) prove the same optimum."""
+    with certified bounds, graphs, warm-started IPMs) prove the same optimum."""
     exe = os.path.join(PKG, "biqbin_gpu")
     r = subprocess.run([exe] + opts + [os.path.join(golden, "rudy", "g05_60.0"), os.path.join(golden, "params")],
                        capture_output=True, text=True, timeout=120)

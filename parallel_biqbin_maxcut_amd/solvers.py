@@ -7,7 +7,7 @@ rudy file or an adjacency JSON (biqbin_base.py:48-89); a QUBO as a matrix or
 a JSON file with a "qubo" entry (biqbin_base.py:186-220).  `params` is the
 path of a parameter file (src/process_input.c:153-180) or a
 BiqBinParameters.  Further keyword arguments select the solver form
-(api.solve: batch, warm, slices, certify, model, graphs).
+(api.solve: batch, warm, ipm_warm, slices, certify, model, graphs).
 
 run() returns the reference's result dictionary: "time", "maxcut" with
 "computed_val" and "solution" (1-based vertices on side 1; the last vertex is

@@ -38,8 +38,10 @@ the same host type, one core each).
 Besides the headline, config carries: time_to_proof (G(60, 0.5) branch and
 bound to the proven optimum: the plain FP64 form one node at a time, a
 64-slot node queue, children inheriting their parent's cuts and multipliers,
-bf16 matrix-core products with certified bounds, and the learned cut model;
-each form's speedup over the plain one), cfg3_mixed (G(250, 0.1), FP64 against
+bf16 matrix-core products with certified bounds, the learned cut model, and
+bundle-step IPMs warm-started from the node's previous solve; each form's
+speedup over the plain one, and the root bound's time alone, cold and warm,
+which caps every form's speedup), cfg3_mixed (G(250, 0.1), FP64 against
 bf16 matrix-core products with certified bounds, with the roofline of the
 mixed IPM against the bf16 peak) and cfg4_qubo (QUBO 150 as 151-vertex
 Max-Cut, 256 nodes per launch).
